@@ -1,0 +1,175 @@
+"""Benchmark: batched forward + adjoint gradient of the time-domain 2D
+acoustic propagator on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode auto|full|ckpt]
+
+Workload (one "step" = one misfit gradient over every shot): the 384 x 1024
+model at h = 12.5 m, dt = 1 ms, nt = 4000, order 8, 32 ricker(8) shots at
+z = 2 with 1024 receivers, eight dipping layers 1500-4500 m/s with a +-5%
+correlated perturbation; observed data are modelled from a second seeded
+perturbation (untimed).  Inputs are resident in HBM before timing starts.
+
+Throughput is Gcell-updates/s with one cell-update = one padded-grid cell of
+one wavefield advanced one step for one shot; a gradient step is
+2 * shots * dimx * dimy * nt cell-updates (checkpoint recomputation is not
+credited).  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0            # MI355X HBM3E, AMD spec
+BYTES_FWD = 16                   # u_prev, u_cur, model read; u_new write
+BYTES_FWD_VIRT = 20              # + virtual-source write
+BYTES_ADJ = 28                   # lambda x2 + model + virt + grad read; lambda + grad write
+BYTES_GRAD_STEP = BYTES_FWD + BYTES_ADJ   # credited per cell*step*shot (checkpointed form)
+METRIC = "Gcell-updates/s fwd+adj FP32 vs HBM roofline; gradient p50/p99 latency"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", default="auto", choices=["auto", "full", "ckpt"])
+    ap.add_argument("--ckpt-every", type=int, default=0, help="K for --mode ckpt (0: ceil(sqrt(nt)))")
+    ap.add_argument("--nshots", type=int, default=32)
+    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="skip per-launch HIP-event timing")
+    return ap.parse_args()
+
+
+def cpu_baseline():
+    """Config-1 forward + gradient on the host cores: the reference's own CPU
+    path when it was built (oracle/_ref), else the oracle's restatement."""
+    from oracle import oracle as O
+    from torchwi_amd import workloads as W
+    cores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    c = W.config1()
+    order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
+    dimx, dimy = nx + order, ny + order
+    vpad = O.pad_vel(c["vel"], order)
+    d_obs, _ = O.forward(O.pad_vel(c["vel_true"], order), c["w"], order, dimx, dimy, h, dt,
+                         c["sx"][0], c["sy"][0], c["ry"][0], store_virt=False)
+    impl = "ref" if O.ref_available() else "oracle"
+
+    def run():
+        frd, virt = O.forward(vpad, c["w"], order, dimx, dimy, h, dt, c["sx"][0], c["sy"][0],
+                              c["ry"][0], impl=impl)
+        O.backward(virt, vpad, frd - d_obs, order, dimx, dimy, h, dt, c["ry"][0], impl=impl)
+
+    run()
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t0)
+    t = statistics.median(ts)
+    cells = 2.0 * dimx * dimy * nt
+    return {"value": round(cells / t / 1e9, 4), "unit": "Gcell-updates/s", "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
+            "kind": "reference" if impl == "ref" else "port",
+            "sample": "100x200 model + 20-cell pad (248x128 padded, order 8), nt=1000, 1 shot, "
+                      "forward with full storage + adjoint gradient; median of 5 runs (%.3f s each)" % t}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box of this build has one MI355X; "
+              "multi-GPU shot sharding is a design in DESIGN.md, not a run (--gpus %d refused)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from torchwi_amd import Time2dBatch, workloads as W
+    from torchwi_amd.batch import default_ckpt_interval
+
+    dev = torch.device("cuda:0")
+    c = W.config2(nshots=args.nshots, nt=args.nt)
+    order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
+    S = len(c["sx"])
+    if args.mode == "full":
+        K = 0
+    elif args.mode == "ckpt":
+        K = args.ckpt_every or default_ckpt_interval(nt)
+    else:
+        K = "auto"
+    model = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K)
+    vel = torch.from_numpy(c["vel"]).to(dev)
+    rng = np.random.default_rng(7)
+    vtrue = torch.from_numpy(c["vel"] * (1.0 + 0.02 * rng.standard_normal(c["vel"].shape)).astype(np.float32)).to(dev)
+    d_obs = model.modeling(vtrue, c["sx"], c["sy"], c["ry"]).clone()
+    st = model._state
+    g = model.g
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        model.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
+    torch.cuda.synchronize()
+    if not args.no_timing:
+        st.set_timing(True)
+        st.timing(reset=True)
+
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        loss, grad = model.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    cells_step = 2.0 * S * g.dimx * g.dimy * nt
+    value = cells_step * args.steps / elapsed / 1e9
+    kstate = st.K
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched misfit gradient (forward + adjoint), %d shots, %dx%d model "
+                               "(padded %dx%d), h=%g m, dt=%g s, nt=%d, order %d, %s"
+                               % (S, ny, nx, g.dimy, g.dimx, h, dt, nt, order,
+                                  "full virtual-source storage" if kstate == 0 else "checkpoint every %d steps + recompute" % kstate),
+                   "shots": S, "nz": ny, "nx": nx, "nt": nt, "order": order, "ckpt_every": kstate},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+        "gradient_shots_per_s": round(S * args.steps / elapsed, 3),
+        "effective_hbm_frac": round(value * 1e9 * (BYTES_GRAD_STEP / 2.0) / (HBM_PEAK_GBS * 1e9), 4),
+    }
+    if not args.no_timing:
+        tm = st.timing()
+        st.set_timing(False)
+        per = {}
+        cells_launch = S * g.dimx * g.dimy
+        for kind, b in (("fwd", BYTES_FWD), ("fwd_virt", BYTES_FWD_VIRT), ("recomp", BYTES_FWD_VIRT), ("adj", BYTES_ADJ)):
+            ms, n = tm[kind]
+            if n:
+                avg = ms / n
+                per[kind] = {"avg_us": round(avg * 1e3, 3), "launches": n,
+                             "achieved_GBs": round(b * cells_launch / (avg * 1e-3) / 1e9, 1)}
+        out["kernels"] = per
+        dom = max(per, key=lambda k: per[k]["avg_us"] * per[k]["launches"])
+        b = {"fwd": BYTES_FWD, "fwd_virt": BYTES_FWD_VIRT, "recomp": BYTES_FWD_VIRT, "adj": BYTES_ADJ}[dom]
+        ach = per[dom]["achieved_GBs"]
+        out["roofline"] = {"bound": "hbm", "kernel": "k_step_batch<4,%s>" % dom, "achieved": ach,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                           "traffic": None, "bytes_per_cell": b, "cells_per_launch": cells_launch}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

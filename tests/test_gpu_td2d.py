@@ -1,0 +1,225 @@
+"""GPU parity of the gfx950 propagator against the CPU oracle.
+
+The HIP kernels reproduce the reference CPU path's mixed-precision
+arithmetic, so the bar here is bit-exact equality with the oracle (which is
+itself bit-exact with the compiled reference, tests/test_oracle.py).  The
+north-star tolerance, relative L2 <= 1e-5, is asserted as well, and the FP32
+error against the FP64 run of the same recursion is printed per result.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.helpers import rel_l2, small_case
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # relative L2 bar of the north star
+
+DEV = "cuda"
+
+
+def _run_module_forward_backward(c, resid=None):
+    from torchwi_amd.propagator import td2d_cuda
+    order, dimx, dimy, nt = c["order"], c["dimx"], c["dimy"], c["nt"]
+    n = dimx * dimy
+    vpad = torch.from_numpy(O.pad_vel(c["vel"], order)).to(DEV)
+    frd = torch.zeros(nt * (dimx - order), device=DEV)
+    virt = torch.zeros(nt * n, device=DEV)
+    u = [torch.zeros(n, device=DEV) for _ in range(3)]
+    w = torch.from_numpy(c["w"])
+    td2d_cuda.forward(frd, virt, *u, vpad, w, order, dimx, dimy, nt, c["h"], c["dt"],
+                      c["sx"], c["sy"], c["ry"])
+    out = dict(frd=frd.cpu().numpy().reshape(nt, -1), virt=virt.cpu().numpy().reshape(nt, n))
+    if resid is not None:
+        g = td2d_cuda.backward(virt, *u, vpad, torch.from_numpy(resid).to(DEV), order, dimx, dimy,
+                               nt, c["h"], c["dt"], c["ry"])
+        out["grad"] = g.cpu().numpy()
+    return out
+
+
+@pytest.mark.parametrize("order", [2, 4, 8])
+@pytest.mark.parametrize("shape", [(37, 23), (150, 70)])
+def test_forward_backward_bitexact(order, shape):
+    nx, ny = shape
+    c = small_case(order, nx=nx, ny=ny, nt=160, seed=order + nx)
+    print(f"case order={order} nx={nx} ny={ny}", flush=True)
+    vpad = O.pad_vel(c["vel"], order)
+    args = (order, c["dimx"], c["dimy"], c["h"], c["dt"])
+    frd_o, virt_o = O.forward(vpad, c["w"], *args, c["sx"], c["sy"], c["ry"])
+    rng = np.random.default_rng(1)
+    resid = rng.standard_normal(frd_o.shape).astype(np.float32)
+    grad_o = O.backward(virt_o, vpad, resid, *args, c["ry"])
+    got = _run_module_forward_backward(c, resid)
+    frd64, virt64 = O.forward64(vpad, c["w"], *args, c["sx"], c["sy"], c["ry"])
+    grad64 = O.backward64(virt64, vpad, resid, *args, c["ry"])
+    print(f"  rel-L2 vs oracle f32: frd {rel_l2(got['frd'], frd_o):.3e} grad {rel_l2(got['grad'], grad_o):.3e}; "
+          f"vs f64: frd {rel_l2(got['frd'], frd64):.3e} grad {rel_l2(got['grad'], grad64):.3e}")
+    assert rel_l2(got["frd"], frd_o) <= TOL and rel_l2(got["grad"], grad_o) <= TOL
+    assert np.array_equal(got["frd"], frd_o)
+    assert np.array_equal(got["virt"], virt_o)
+    assert np.array_equal(got["grad"], grad_o)
+
+
+@pytest.mark.parametrize("order", [2, 4, 8])
+def test_forward_only_bitexact(order):
+    from torchwi_amd.propagator import td2d_forward_cuda
+    c = small_case(order, nx=90, ny=41, nt=150, seed=7)
+    vpad = O.pad_vel(c["vel"], order)
+    frd_o, _ = O.forward(vpad, c["w"], order, c["dimx"], c["dimy"], c["h"], c["dt"],
+                         c["sx"], c["sy"], c["ry"], store_virt=False)
+    n = c["dimx"] * c["dimy"]
+    frd = torch.zeros(c["nt"] * c["nx"], device=DEV)
+    u = [torch.zeros(n, device=DEV) for _ in range(3)]
+    td2d_forward_cuda.forward(frd, *u, torch.from_numpy(vpad).to(DEV), torch.from_numpy(c["w"]),
+                              order, c["dimx"], c["dimy"], c["nt"], c["h"], c["dt"],
+                              c["sx"], c["sy"], c["ry"])
+    assert np.array_equal(frd.cpu().numpy().reshape(c["nt"], -1), frd_o)
+
+
+@pytest.mark.parametrize("order", [4, 8])
+def test_exa_bitexact(order):
+    from torchwi_amd.propagator import td2d_exa_cuda
+    c = small_case(order, nx=70, ny=33, nt=140, seed=11)
+    vpad = O.pad_vel(c["vel"], order)
+    args = (order, c["dimx"], c["dimy"], c["h"], c["dt"])
+    frd_o, exa_o, iexa_o = O.exa_forward(vpad, c["w"], *args, c["sx"], c["sy"], c["ry"])
+    resid = np.random.default_rng(2).standard_normal(frd_o.shape).astype(np.float32)
+    grad_o = O.exa_backward(exa_o, iexa_o, vpad, resid, *args, c["ry"])
+    n = c["dimx"] * c["dimy"]
+    frd = torch.zeros(c["nt"] * c["nx"], device=DEV)
+    exa = torch.zeros(n, device=DEV)
+    iexa = torch.zeros(n, dtype=torch.int16, device=DEV)
+    u = [torch.zeros(n, device=DEV) for _ in range(3)]
+    vp = torch.from_numpy(vpad).to(DEV)
+    td2d_exa_cuda.forward(frd, exa, iexa, *u, vp, torch.from_numpy(c["w"]), order, c["dimx"],
+                          c["dimy"], c["nt"], c["h"], c["dt"], c["sx"], c["sy"], c["ry"])
+    g = td2d_exa_cuda.backward(exa, iexa, *u, vp, torch.from_numpy(resid).to(DEV), order,
+                               c["dimx"], c["dimy"], c["nt"], c["h"], c["dt"], c["ry"])
+    assert np.array_equal(frd.cpu().numpy().reshape(c["nt"], -1), frd_o)
+    assert np.array_equal(exa.cpu().numpy(), exa_o)
+    assert np.array_equal(iexa.cpu().numpy(), iexa_o)
+    assert np.array_equal(g.cpu().numpy(), grad_o)
+
+
+def test_time2d_autograd_matches_oracle():
+    """The drop-in module surface: Time2d forward + loss.backward()."""
+    import torchwi_amd as torchwi
+    order = 8
+    c = small_case(order, nx=64, ny=40, nt=180, seed=3)
+    vpad = O.pad_vel(c["vel"], order)
+    args = (order, c["dimx"], c["dimy"], c["h"], c["dt"])
+    frd_o, virt_o = O.forward(vpad, c["w"], *args, c["sx"], c["sy"], c["ry"])
+    true = (0.9 * frd_o).astype(np.float32)
+    grad_o = O.crop_grad(O.backward(virt_o, vpad, frd_o - true, *args, c["ry"]), order, c["nx"], c["ny"])
+
+    model = torchwi.Time2d(c["nx"], c["ny"], c["h"], torch.from_numpy(c["w"]), c["dt"], order, DEV)
+    vel = torch.nn.Parameter(torch.from_numpy(c["vel"]).to(DEV))
+    sx, sy, ry = (torch.tensor([c[k]]) for k in ("sx", "sy", "ry"))
+    frd = model(vel, sx, sy, ry)
+    assert tuple(frd.shape) == (c["nt"], c["nx"])
+    loss = 0.5 * torch.sum((frd - torch.from_numpy(true).to(DEV)) ** 2)
+    loss.backward()
+    assert tuple(vel.grad.shape) == (c["nx"], c["ny"])
+    assert np.array_equal(frd.detach().cpu().numpy(), frd_o)
+    assert np.array_equal(vel.grad.cpu().numpy(), grad_o)
+
+
+def test_time2dforward_module():
+    import torchwi_amd as torchwi
+    order = 4
+    c = small_case(order, nx=50, ny=30, nt=100, seed=4)
+    vpad = O.pad_vel(c["vel"], order)
+    frd_o, _ = O.forward(vpad, c["w"], order, c["dimx"], c["dimy"], c["h"], c["dt"],
+                         c["sx"], c["sy"], c["ry"], store_virt=False)
+    m = torchwi.Time2dForward(c["nx"], c["ny"], c["h"], torch.from_numpy(c["w"]), c["dt"], order, DEV)
+    with torch.no_grad():
+        frd = m(torch.from_numpy(c["vel"]).to(DEV), torch.tensor([c["sx"]]), torch.tensor([c["sy"]]),
+                torch.tensor([c["ry"]]))
+    assert np.array_equal(frd.cpu().numpy(), frd_o)
+
+
+@pytest.mark.parametrize("order", [2, 8])
+def test_batch_matches_single_and_checkpoint_matches_full(order):
+    """Batched lockstep shots equal one-shot calls bit for bit, and the
+    checkpointed gradient equals the full-storage gradient bit for bit."""
+    from torchwi_amd import Time2dBatch
+    c = small_case(order, nx=100, ny=45, nt=230, seed=5)
+    sx = [(i + 0.5) * c["h"] for i in (5, 33, 61, 94)]
+    sy = [(2 + 0.5) * c["h"]] * 4
+    ry = [(3 + 0.5) * c["h"]] * 4
+    vel = torch.from_numpy(c["vel"]).to(DEV)
+    w = torch.from_numpy(c["w"])
+    full = Time2dBatch(c["nx"], c["ny"], c["h"], w, c["dt"], order, DEV, ckpt_every=0)
+    d_obs = torch.zeros(4, c["nt"], c["nx"], device=DEV)
+    frd_full = full.modeling(vel, sx, sy, ry).clone()
+    d_obs.copy_(0.8 * frd_full)
+    loss_f, g_full = full.misfit_gradient(vel, sx, sy, ry, d_obs)
+    g_full = g_full.clone()
+    per_shot_full = full._state.grad.clone()
+    for K in (7, 64, 1000):
+        ck = Time2dBatch(c["nx"], c["ny"], c["h"], w, c["dt"], order, DEV, ckpt_every=K)
+        loss_c, g_ck = ck.misfit_gradient(vel, sx, sy, ry, d_obs)
+        assert torch.equal(ck._state.frd, frd_full)
+        assert torch.equal(g_ck, g_full), f"K={K}"
+        assert float(loss_c) == float(loss_f)
+    # each shot of the batch equals the one-shot oracle
+    vpad = O.pad_vel(c["vel"], order)
+    args = (order, c["dimx"], c["dimy"], c["h"], c["dt"])
+    g = full.g
+    for s in range(4):
+        frd_o, virt_o = O.forward(vpad, c["w"], *args, sx[s], sy[s], ry[s])
+        assert np.array_equal(frd_full[s].cpu().numpy(), frd_o)
+        resid = frd_o - 0.8 * frd_o
+        resid = (frd_full[s] - d_obs[s]).cpu().numpy()
+        grad_o = O.backward(virt_o, vpad, resid, *args, ry[s])
+        got = per_shot_full[s].view(g.dimy, g.ld)[:, :g.dimx].cpu().numpy().ravel()
+        assert np.array_equal(got, grad_o)
+
+
+def test_config1_forward_gradient_bitexact():
+    """The reference's CPU-runnable case: 100x200 model behind a 20-cell pad,
+    nt=1000, order 8, gradient of the L2 misfit against the blob model."""
+    from torchwi_amd import workloads as W
+    import torchwi_amd as torchwi
+    c = W.config1()
+    order, nx, ny, nt = c["order"], c["nx"], c["ny"], c["nt"]
+    dimx, dimy = nx + order, ny + order
+    vpad = O.pad_vel(c["vel"], order)
+    vpad_t = O.pad_vel(c["vel_true"], order)
+    a = (order, dimx, dimy, c["h"], c["dt"], c["sx"][0], c["sy"][0], c["ry"][0])
+    d_obs, _ = O.forward(vpad_t, c["w"], *a, store_virt=False)
+    frd_o, virt_o = O.forward(vpad, c["w"], *a)
+    grad_o = O.crop_grad(O.backward(virt_o, vpad, frd_o - d_obs, order, dimx, dimy, c["h"], c["dt"],
+                                    c["ry"][0]), order, nx, ny)
+    frd64, virt64 = O.forward64(vpad, c["w"], *a)
+    d64, _ = O.forward64(vpad_t, c["w"], *a, store_virt=False)
+    grad64 = O.crop_grad(O.backward64(virt64, vpad, frd64 - d64, order, dimx, dimy, c["h"], c["dt"],
+                                      c["ry"][0]), order, nx, ny)
+
+    model = torchwi.Time2d(nx, ny, c["h"], torch.from_numpy(c["w"]), c["dt"], order, DEV)
+    vel = torch.nn.Parameter(torch.from_numpy(c["vel"]).to(DEV))
+    frd = model(vel, torch.tensor([c["sx"][0]]), torch.tensor([c["sy"][0]]), torch.tensor([c["ry"][0]]))
+    loss = 0.5 * torch.sum((frd - torch.from_numpy(d_obs).to(DEV)) ** 2)
+    loss.backward()
+    f, g = frd.detach().cpu().numpy(), vel.grad.cpu().numpy()
+    print(f"config1: rel-L2 vs oracle frd {rel_l2(f, frd_o):.2e} grad {rel_l2(g, grad_o):.2e}; "
+          f"vs FP64 frd {rel_l2(f, frd64):.2e} grad {rel_l2(g, grad64):.2e}")
+    assert np.array_equal(f, frd_o)
+    assert np.array_equal(g, grad_o)
+
+
+def test_invalid_arguments_raise():
+    from torchwi_amd.propagator import td2d_cuda
+    n = 20 * 20
+    z = lambda k: torch.zeros(k, device=DEV)  # noqa: E731
+    with pytest.raises(RuntimeError, match="order"):
+        td2d_cuda.forward(z(100), z(100 * n), z(n), z(n), z(n), z(n) + 2000, torch.zeros(10),
+                          6, 20, 20, 10, 10.0, 1e-3, 50.0, 20.0, 20.0)
+    with pytest.raises(RuntimeError, match="source"):
+        td2d_cuda.forward(z(10 * 12), z(10 * n), z(n), z(n), z(n), z(n) + 2000, torch.zeros(10),
+                          8, 20, 20, 10, 10.0, 1e-3, 5000.0, 20.0, 20.0)
+    with pytest.raises(RuntimeError, match="elements"):
+        td2d_cuda.forward(z(5), z(10 * n), z(n), z(n), z(n), z(n) + 2000, torch.zeros(10),
+                          8, 20, 20, 10, 10.0, 1e-3, 50.0, 20.0, 20.0)

@@ -1,0 +1,891 @@
+// td2d_hip.hip -- gfx950 kernels and C ABI of the time-domain 2D acoustic
+// propagator (forward modelling, adjoint imaging-condition gradient, EXA,
+// batched shots with checkpointing).
+//
+// One launch advances one time step of every shot in the launch.  Each
+// workgroup (256 threads, 4 waves of 64) owns a TX x TY tile of one shot's
+// padded grid: it stages the current wavefield with its ne-wide halo in LDS
+// with x-contiguous (coalesced) loads, then every thread updates RY cells of
+// one column.  Every cell of the padded grid is written by exactly one
+// thread with the formula of its region (interior stencil, Keys on the
+// left/right/bottom strips, zero on the top free-surface rows), so the Keys
+// boundary, source injection, receiver sampling, virtual-source storage and
+// gradient accumulation are all fused into the same step.
+//
+// Compile with -ffp-contract=off and correctly rounded fp32 division: the
+// per-cell arithmetic reproduces the reference CPU path bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "td2d_arith.hpp"
+#include "torchwi_hip.h"
+
+namespace tw {
+
+constexpr int TX = 64;      // tile width (one wavefront of columns)
+constexpr int TY = 32;      // tile height
+constexpr int RY = 8;       // rows per thread
+constexpr int NTHR = TX * (TY / RY);
+constexpr int NXCD = 8;     // MI355X: 8 XCDs, workgroups dealt round-robin
+
+enum : unsigned {
+    F_SRC = 1u,       // inject the point source w[it]
+    F_REC = 2u,       // sample the receiver row into frd[it]
+    F_VIRT = 4u,      // store the virtual source into virt slot
+    F_RESID = 8u,     // inject the residual row resid[it] (adjoint)
+    F_GRAD = 16u,     // grad += -virt[slot] * u_new
+    F_EXA_FWD = 32u,  // excitation-amplitude update
+    F_EXA_ADJ = 64u,  // excitation-amplitude imaging condition
+};
+
+// Device-side description of one shot.
+struct ShotDesc {
+    int dimx, dimy, ld;
+    int tiles_x, tile0;
+    int rrow, rx0, nrec;      // receiver row / window in padded coordinates
+    long long isrc;           // linear (ld-strided) index of the source cell
+    long long fstride;        // floats between consecutive fields of a ring
+    float dt2, dtoh2, h2, hdt, cons;
+    const float* vel;
+    const float* w;
+    float* u[3];              // forward ring
+    float* lam[3];            // adjoint ring
+    float* frd;
+    const float* resid;
+    float* virt;
+    float* ckpt;
+    float* grad;
+    float* exa;
+    short* iexa;
+};
+
+struct StepArgs {
+    int it;          // time index of this step
+    int ring;        // rotation: u_prev = ring%3, u_cur = (ring+1)%3, u_new = (ring+2)%3
+    int vslot;       // virt slot for F_VIRT / F_GRAD
+    int ck0, ck1;    // checkpoint segment receiving u_new in slot 0 / 1 (-1: none)
+    unsigned flags;
+    int adj;         // 1: step the adjoint ring
+    int ntiles;      // tiles in the launch
+    int per_xcd;     // ceil(ntiles / NXCD)
+};
+
+// Map a hardware workgroup id to a logical tile so that each XCD receives a
+// contiguous band of tiles (shared halos stay in that XCD's L2).
+__device__ __forceinline__ int logical_tile(int b, int per_xcd)
+{
+    return (b % NXCD) * per_xcd + b / NXCD;
+}
+
+// Select one of three ring pointers without dynamic indexing (which would
+// force a by-value descriptor into scratch memory).
+__device__ __forceinline__ float* pick3(float* const (&r)[3], int k)
+{
+    return k == 0 ? r[0] : (k == 1 ? r[1] : r[2]);
+}
+
+template <int NE, unsigned FLAGS>
+__device__ __forceinline__ void step_tile(const ShotDesc& d, int lt, const StepArgs& a)
+{
+    constexpr int W = TX + 2 * NE;
+    constexpr int H = TY + 2 * NE;
+    __shared__ float T[H * W];
+
+    const int by = lt / d.tiles_x;
+    const int bx = lt - by * d.tiles_x;
+    const int x0 = bx * TX, y0 = by * TY;
+    const int dimx = d.dimx, dimy = d.dimy;
+    const long long ld = d.ld;
+
+    const int k0 = a.ring % 3, k1 = (a.ring + 1) % 3, k2 = (a.ring + 2) % 3;
+    const float* __restrict__ um = a.adj ? pick3(d.lam, k0) : pick3(d.u, k0);
+    const float* __restrict__ uo = a.adj ? pick3(d.lam, k1) : pick3(d.u, k1);
+    float* __restrict__ up = a.adj ? pick3(d.lam, k2) : pick3(d.u, k2);
+
+    // Stage u_cur with its halo.  Cells outside the padded grid are never
+    // read by an interior stencil; they are filled with zero.
+    for (int k = threadIdx.x; k < H * W; k += NTHR) {
+        const int r = k / W;
+        const int c = k - r * W;
+        const int gy = y0 - NE + r, gx = x0 - NE + c;
+        float val = 0.f;
+        if (gy >= 0 && gy < dimy && gx >= 0 && gx < dimx) val = uo[gy * ld + gx];
+        T[k] = val;
+    }
+    __syncthreads();
+
+    const int tx = threadIdx.x % TX;
+    const int gx = x0 + tx;
+    if (gx >= dimx) return;
+    const int ly0 = (threadIdx.x / TX) * RY;
+    constexpr unsigned flags = FLAGS;
+
+    for (int r = 0; r < RY; ++r) {
+        const int ly = ly0 + r;
+        const int gy = y0 + ly;
+        if (gy >= dimy) break;
+        const long long i = gy * ld + gx;
+        const float* Tc = &T[(ly + NE) * W + tx + NE];
+        const float vv = d.vel[i];
+        const float um0 = um[i];
+        const float uo0 = Tc[0];
+        float un;
+        if (gy < NE) {
+            un = 0.f;  // free surface: the reference never writes these rows
+        } else if (gy >= dimy - NE) {
+            un = keys_update(um0, uo0, um[i - ld], uo[i - ld], uo[i - 2 * ld], vv, d.h2, d.dt2, d.hdt);
+        } else if (gx < NE) {
+            un = keys_update(um0, uo0, um[i + 1], uo[i + 1], uo[i + 2], vv, d.h2, d.dt2, d.hdt);
+        } else if (gx >= dimx - NE) {
+            un = keys_update(um0, uo0, um[i - 1], uo[i - 1], uo[i - 2], vv, d.h2, d.dt2, d.hdt);
+        } else {
+            un = interior_update<NE>(Tc, W, um0, (vv * vv) * d.dtoh2);
+        }
+        const bool on_rec = gy == d.rrow && gx >= d.rx0 && gx < d.rx0 + d.nrec;
+        if ((flags & F_SRC) && i == d.isrc) un += inject_term(d.w[a.it], vv, d.dt2);
+        if ((flags & F_RESID) && on_rec)
+            un += inject_term(d.resid[(long long)a.it * d.nrec + (gx - d.rx0)], vv, d.dt2);
+        up[i] = un;
+        if ((flags & F_REC) && on_rec) d.frd[(long long)a.it * d.nrec + (gx - d.rx0)] = un;
+        if (flags & F_VIRT) d.virt[a.vslot * d.fstride + i] = virt_source(d.cons, vv, um0, uo0, un);
+        if (flags & F_GRAD) {
+            const float vs = d.virt[a.vslot * d.fstride + i];
+            d.grad[i] = d.grad[i] + (-vs) * un;
+        }
+        if (flags & F_EXA_FWD) {
+            const float vs = virt_source_exa(d.cons, vv, um0, uo0, un);
+            if (fabsf(vs) > fabsf(d.exa[i])) {
+                d.exa[i] = vs;
+                d.iexa[i] = (short)a.it;
+            }
+        }
+        if ((flags & F_EXA_ADJ) && d.iexa[i] == a.it) d.grad[i] = (-d.exa[i]) * un;
+        if (a.ck0 >= 0) d.ckpt[(2LL * a.ck0) * d.fstride + i] = un;
+        if (a.ck1 >= 0) d.ckpt[(2LL * a.ck1 + 1) * d.fstride + i] = un;
+    }
+}
+
+// Step variants (compile-time flag sets).
+constexpr unsigned M_FWD = F_SRC | F_REC;             // forward modelling (+checkpoints)
+constexpr unsigned M_FWD_VIRT = F_SRC | F_REC | F_VIRT; // forward with full storage
+constexpr unsigned M_RECOMP = F_SRC | F_VIRT;         // checkpoint-segment recomputation
+constexpr unsigned M_ADJ = F_RESID | F_GRAD;          // adjoint + imaging condition
+constexpr unsigned M_EXA_FWD = F_SRC | F_REC | F_EXA_FWD;
+constexpr unsigned M_EXA_ADJ = F_RESID | F_EXA_ADJ;
+
+template <int NE, unsigned FLAGS>
+__global__ __launch_bounds__(NTHR) void k_step_single(ShotDesc d, StepArgs a)
+{
+    const int lt = logical_tile(blockIdx.x, a.per_xcd);
+    if (lt >= a.ntiles) return;
+    step_tile<NE, FLAGS>(d, lt, a);
+}
+
+template <int NE, unsigned FLAGS>
+__global__ __launch_bounds__(NTHR) void k_step_batch(const ShotDesc* __restrict__ descs,
+                                                     const int* __restrict__ tile_shot,
+                                                     StepArgs a)
+{
+    const int t = logical_tile(blockIdx.x, a.per_xcd);
+    if (t >= a.ntiles) return;
+    const int s = tile_shot[t];
+    const ShotDesc& d = descs[s];
+    step_tile<NE, FLAGS>(d, t - d.tile0, a);
+}
+
+// Zero n floats (a fill kernel, not hipMemsetAsync, so the call can be
+// captured into a graph).
+__global__ void k_zero(float* __restrict__ p, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0.f;
+}
+
+__global__ void k_zero_i16(short* __restrict__ p, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// Per shot (blockIdx.y), prepare the forward ring for a segment starting at
+// step t0: u_prev <- ckpt[seg][0], u_cur <- ckpt[seg][1] (zero for seg 0);
+// or (what = 1) zero the whole adjoint ring; or (what = 2) zero grad.
+__global__ void k_batch_prepare(const ShotDesc* __restrict__ descs, int what, int seg, int t0)
+{
+    const ShotDesc& d = descs[blockIdx.y];
+    const long long n = (long long)d.dimy * d.ld;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (what == 0) {
+        float* p0 = pick3(d.u, t0 % 3);
+        float* p1 = pick3(d.u, (t0 + 1) % 3);
+        if (seg == 0) {
+            for (long long i = i0; i < n; i += stride) { p0[i] = 0.f; p1[i] = 0.f; }
+        } else {
+            const float* c0 = d.ckpt + (2LL * seg) * d.fstride;
+            const float* c1 = c0 + d.fstride;
+            for (long long i = i0; i < n; i += stride) { p0[i] = c0[i]; p1[i] = c1[i]; }
+        }
+    } else if (what == 1) {
+        for (long long i = i0; i < n; i += stride) {
+            d.lam[0][i] = 0.f; d.lam[1][i] = 0.f; d.lam[2][i] = 0.f;
+        }
+    } else {
+        for (long long i = i0; i < n; i += stride) d.grad[i] = 0.f;
+    }
+}
+
+// ------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define TW_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(TW_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));      \
+    } while (0)
+
+// Physical constants of a grid, computed as the reference computes them in
+// float (propagator_cpu.cpp:36-41).
+void fill_consts(ShotDesc& d, float h, float dt)
+{
+    d.dt2 = dt * dt;
+    d.h2 = h * h;
+    d.dtoh2 = d.dt2 / d.h2;
+    d.hdt = h * dt;
+    d.cons = (float)(-2.0 / (double)d.dt2);
+}
+
+// Geometry truncation exactly as the reference (propagator_cpu.cpp:44-48).
+struct Cells { int isx, isy, iry; };
+Cells truncate_positions(float sx, float sy, float ry, float h)
+{
+    Cells c;
+    c.isx = (int)(sx / h);
+    c.isy = (int)(sy / h);
+    c.iry = (int)(ry / h);
+    return c;
+}
+
+int check_grid(int order, int dimx, int dimy, int nt, float h, float dt)
+{
+    if (order != 2 && order != 4 && order != 8)
+        return fail(TW_EINVAL, "order must be 2, 4 or 8 (got %d)", order);
+    if (dimx <= order || dimy <= order)
+        return fail(TW_EINVAL, "grid %dx%d has no interior for order %d", dimx, dimy, order);
+    if ((long long)dimx * dimy > (1LL << 40))
+        return fail(TW_EINVAL, "grid %dx%d too large", dimx, dimy);
+    if (nt <= 0) return fail(TW_EINVAL, "nt must be positive (got %d)", nt);
+    if (!(h > 0.f) || !(dt > 0.f)) return fail(TW_EINVAL, "h and dt must be positive");
+    return TW_OK;
+}
+
+int check_src(const Cells& c, int order, int dimx, int dimy)
+{
+    const int nx = dimx - order, ny = dimy - order;
+    if (c.isx < 0 || c.isx >= nx || c.isy < 0 || c.isy >= ny)
+        return fail(TW_EINVAL, "source cell (isx=%d, isy=%d) outside the %dx%d interior", c.isx,
+                    c.isy, nx, ny);
+    return TW_OK;
+}
+
+int check_rec(const Cells& c, int order, int dimy)
+{
+    const int ny = dimy - order;
+    if (c.iry < 0 || c.iry >= ny)
+        return fail(TW_EINVAL, "receiver row iry=%d outside the interior (ny=%d)", c.iry, ny);
+    return TW_OK;
+}
+
+void grid_tiles(int dimx, int dimy, int& tx, int& ty)
+{
+    tx = (dimx + TX - 1) / TX;
+    ty = (dimy + TY - 1) / TY;
+}
+
+template <int NE, unsigned FLAGS>
+void launch_single_nf(const ShotDesc& d, const StepArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_step_single<NE, FLAGS>), dim3(a.per_xcd * NXCD), dim3(NTHR), 0, s, d, a);
+}
+
+template <int NE>
+int launch_single_ne(const ShotDesc& d, const StepArgs& a, hipStream_t s)
+{
+    switch (a.flags) {
+    case M_FWD: launch_single_nf<NE, M_FWD>(d, a, s); break;
+    case M_FWD_VIRT: launch_single_nf<NE, M_FWD_VIRT>(d, a, s); break;
+    case M_ADJ: launch_single_nf<NE, M_ADJ>(d, a, s); break;
+    case M_EXA_FWD: launch_single_nf<NE, M_EXA_FWD>(d, a, s); break;
+    case M_EXA_ADJ: launch_single_nf<NE, M_EXA_ADJ>(d, a, s); break;
+    default: return fail(TW_EINVAL, "internal: no single-shot step variant for flags %u", a.flags);
+    }
+    return TW_OK;
+}
+
+int launch_single(int order, const ShotDesc& d, StepArgs a, hipStream_t s)
+{
+    int rc;
+    if (order == 2) rc = launch_single_ne<1>(d, a, s);
+    else if (order == 4) rc = launch_single_ne<2>(d, a, s);
+    else rc = launch_single_ne<4>(d, a, s);
+    if (rc) return rc;
+    TW_HIP(hipGetLastError());
+    return TW_OK;
+}
+
+int zero_f(float* p, long long n, hipStream_t s)
+{
+    if (n <= 0) return TW_OK;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    TW_HIP(hipGetLastError());
+    return TW_OK;
+}
+
+int zero_i16(short* p, long long n, hipStream_t s)
+{
+    if (n <= 0) return TW_OK;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_zero_i16, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    TW_HIP(hipGetLastError());
+    return TW_OK;
+}
+
+// Common set-up of a single-shot descriptor (ld = dimx, fields of dimx*dimy,
+// receivers on every interior column of row iry).
+int single_desc(ShotDesc& d, int order, int dimx, int dimy, int nt, float h, float dt,
+                float sx, float sy, float ry, bool need_src, const float* vpad)
+{
+    int rc = check_grid(order, dimx, dimy, nt, h, dt);
+    if (rc) return rc;
+    if (!vpad) return fail(TW_EINVAL, "vpad is NULL");
+    const int ne = order / 2;
+    Cells c = truncate_positions(sx, sy, ry, h);
+    if (need_src && (rc = check_src(c, order, dimx, dimy))) return rc;
+    if ((rc = check_rec(c, order, dimy))) return rc;
+    std::memset(&d, 0, sizeof d);
+    d.dimx = dimx;
+    d.dimy = dimy;
+    d.ld = dimx;
+    int tyy;
+    grid_tiles(dimx, dimy, d.tiles_x, tyy);
+    d.tile0 = 0;
+    d.rrow = c.iry + ne;
+    d.rx0 = ne;
+    d.nrec = dimx - order;
+    d.isrc = need_src ? (long long)(c.isy + ne) * dimx + (c.isx + ne) : -1;
+    d.fstride = (long long)dimx * dimy;
+    fill_consts(d, h, dt);
+    d.vel = vpad;
+    return TW_OK;
+}
+
+StepArgs base_args(const ShotDesc& d)
+{
+    StepArgs a;
+    std::memset(&a, 0, sizeof a);
+    int tx, ty;
+    grid_tiles(d.dimx, d.dimy, tx, ty);
+    a.ntiles = tx * ty;
+    a.per_xcd = (a.ntiles + NXCD - 1) / NXCD;
+    a.ck0 = a.ck1 = -1;
+    return a;
+}
+
+// Forward time loop shared by the three single-shot forward entry points.
+int run_forward_single(ShotDesc& d, int order, int nt, unsigned flags, hipStream_t s)
+{
+    int rc;
+    const long long n = d.fstride;
+    for (int k = 0; k < 3; ++k)
+        if ((rc = zero_f(d.u[k], n, s))) return rc;
+    StepArgs a = base_args(d);
+    a.flags = flags;
+    for (int it = 0; it < nt; ++it) {
+        a.it = it;
+        a.ring = it;
+        a.vslot = it;
+        if ((rc = launch_single(order, d, a, s))) return rc;
+    }
+    return TW_OK;
+}
+
+int run_adjoint_single(ShotDesc& d, int order, int nt, unsigned flags, hipStream_t s)
+{
+    int rc;
+    const long long n = d.fstride;
+    for (int k = 0; k < 3; ++k)
+        if ((rc = zero_f(d.lam[k], n, s))) return rc;
+    StepArgs a = base_args(d);
+    a.flags = flags;
+    a.adj = 1;
+    for (int it = nt - 1; it >= 0; --it) {
+        a.it = it;
+        a.ring = nt - 1 - it;
+        a.vslot = it;
+        if ((rc = launch_single(order, d, a, s))) return rc;
+    }
+    return TW_OK;
+}
+
+// ------------------------------------------------------------------------
+// batched plans
+// ------------------------------------------------------------------------
+
+// Optional per-launch HIP-event timing of a plan's step kernels: a ring of
+// event pairs; pairs are harvested (accumulated per kernel kind) when the ring
+// wraps and when the caller collects.
+struct EventPair {
+    hipEvent_t start = nullptr, stop = nullptr;
+    int kind = -1;
+};
+
+struct Timer {
+    bool enabled = false;
+    std::vector<EventPair> ring;
+    size_t head = 0, pending = 0;
+    double ms[TW_NKINDS] = {0};
+    long long count[TW_NKINDS] = {0};
+
+    int init(size_t n)
+    {
+        if (!ring.empty()) return TW_OK;
+        ring.resize(n);
+        for (auto& e : ring) {
+            if (hipEventCreate(&e.start) != hipSuccess || hipEventCreate(&e.stop) != hipSuccess)
+                return fail(TW_EHIP, "hipEventCreate failed");
+        }
+        return TW_OK;
+    }
+    void harvest_one(EventPair& e)
+    {
+        if (e.kind >= 0) {
+            float t = 0.f;
+            if (hipEventSynchronize(e.stop) == hipSuccess &&
+                hipEventElapsedTime(&t, e.start, e.stop) == hipSuccess) {
+                ms[e.kind] += t;
+                count[e.kind] += 1;
+            }
+            e.kind = -1;
+        }
+    }
+    EventPair* next(hipStream_t)
+    {
+        EventPair& e = ring[head];
+        harvest_one(e);  // the slot's previous pair (ring wrapped)
+        head = (head + 1) % ring.size();
+        return &e;
+    }
+    void harvest_all()
+    {
+        for (auto& e : ring) harvest_one(e);
+    }
+    void reset()
+    {
+        harvest_all();
+        for (int k = 0; k < TW_NKINDS; ++k) { ms[k] = 0; count[k] = 0; }
+    }
+    ~Timer()
+    {
+        for (auto& e : ring) {
+            if (e.start) (void)hipEventDestroy(e.start);
+            if (e.stop) (void)hipEventDestroy(e.stop);
+        }
+    }
+};
+
+struct Plan {
+    Timer timer;
+    int order = 0, nt = 0, K = 0, nseg = 0, nshots = 0, ntiles = 0, per_xcd = 0;
+    long long max_field = 0;
+    std::vector<ShotDesc> host;
+    std::vector<int> tile_shot;
+    ShotDesc* d_desc = nullptr;
+    int* d_tile_shot = nullptr;
+};
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan& p)
+{
+    if (!spec || !shots || nshots <= 0) return fail(TW_EINVAL, "empty shot list");
+    int rc = TW_OK;
+    p.order = spec->order;
+    p.nt = spec->nt;
+    p.K = spec->ckpt_every;
+    if (p.K < 0) return fail(TW_EINVAL, "ckpt_every must be >= 0");
+    p.nseg = p.K > 0 ? (p.nt + p.K - 1) / p.K : 0;
+    p.nshots = nshots;
+    const int ne = spec->order / 2;
+    p.host.resize(nshots);
+    p.tile_shot.clear();
+    for (int s = 0; s < nshots; ++s) {
+        const TwShotSpec& q = shots[s];
+        if ((rc = check_grid(spec->order, q.dimx, q.dimy, spec->nt, spec->h, spec->dt))) return rc;
+        if (q.ld < q.dimx) return fail(TW_EINVAL, "shot %d: ld %d < dimx %d", s, q.ld, q.dimx);
+        Cells c = truncate_positions(q.sx, q.sy, q.ry, spec->h);
+        if ((rc = check_src(c, spec->order, q.dimx, q.dimy))) return rc;
+        if ((rc = check_rec(c, spec->order, q.dimy))) return rc;
+        const int nx = q.dimx - spec->order;
+        if (q.rx0 < 0 || q.nrec <= 0 || q.rx0 + q.nrec > nx)
+            return fail(TW_EINVAL, "shot %d: receiver window [%d,%d) outside [0,%d)", s, q.rx0,
+                        q.rx0 + q.nrec, nx);
+        if (!q.vel || !q.w || !q.u || !q.lam || !q.frd || !q.grad || !q.virt)
+            return fail(TW_EINVAL, "shot %d: NULL buffer", s);
+        if (p.K > 0 && !q.ckpt) return fail(TW_EINVAL, "shot %d: NULL checkpoint buffer", s);
+        if (p.K > 0 && q.lam == q.u)
+            return fail(TW_EINVAL, "shot %d: checkpointing needs separate forward and adjoint rings", s);
+        ShotDesc& d = p.host[s];
+        std::memset(&d, 0, sizeof d);
+        d.dimx = q.dimx;
+        d.dimy = q.dimy;
+        d.ld = q.ld;
+        int ty;
+        grid_tiles(q.dimx, q.dimy, d.tiles_x, ty);
+        d.tile0 = (int)p.tile_shot.size();
+        for (int k = 0; k < d.tiles_x * ty; ++k) p.tile_shot.push_back(s);
+        d.rrow = c.iry + ne;
+        d.rx0 = q.rx0 + ne;
+        d.nrec = q.nrec;
+        d.isrc = (long long)(c.isy + ne) * q.ld + (c.isx + ne);
+        d.fstride = (long long)q.dimy * q.ld;
+        if (d.fstride > p.max_field) p.max_field = d.fstride;
+        fill_consts(d, spec->h, spec->dt);
+        d.vel = q.vel;
+        d.w = q.w;
+        for (int k = 0; k < 3; ++k) {
+            d.u[k] = q.u + k * d.fstride;
+            d.lam[k] = q.lam + k * d.fstride;
+        }
+        d.frd = q.frd;
+        d.resid = q.resid;
+        d.virt = q.virt;
+        d.ckpt = q.ckpt;
+        d.grad = q.grad;
+    }
+    p.ntiles = (int)p.tile_shot.size();
+    p.per_xcd = (p.ntiles + NXCD - 1) / NXCD;
+    return TW_OK;
+}
+
+size_t plan_bytes(const Plan& p)
+{
+    return align256(sizeof(ShotDesc) * p.nshots) + align256(sizeof(int) * p.ntiles);
+}
+
+template <int NE, unsigned FLAGS>
+void launch_batch_nf(const Plan& p, const StepArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_step_batch<NE, FLAGS>), dim3(p.per_xcd * NXCD), dim3(NTHR), 0, s,
+                       p.d_desc, p.d_tile_shot, a);
+}
+
+template <int NE>
+int launch_batch_ne(const Plan& p, const StepArgs& a, hipStream_t s)
+{
+    switch (a.flags) {
+    case M_FWD: launch_batch_nf<NE, M_FWD>(p, a, s); break;
+    case M_FWD_VIRT: launch_batch_nf<NE, M_FWD_VIRT>(p, a, s); break;
+    case M_RECOMP: launch_batch_nf<NE, M_RECOMP>(p, a, s); break;
+    case M_ADJ: launch_batch_nf<NE, M_ADJ>(p, a, s); break;
+    default: return fail(TW_EINVAL, "internal: no batch step variant for flags %u", a.flags);
+    }
+    return TW_OK;
+}
+
+// Kernel kinds timed by the plan's optional HIP-event instrumentation.
+int timing_kind(unsigned flags)
+{
+    switch (flags) {
+    case M_FWD: return TW_KIND_FWD;
+    case M_FWD_VIRT: return TW_KIND_FWD_VIRT;
+    case M_RECOMP: return TW_KIND_RECOMP;
+    case M_ADJ: return TW_KIND_ADJ;
+    default: return -1;
+    }
+}
+
+int launch_batch(Plan& p, StepArgs a, hipStream_t s)
+{
+    a.ntiles = p.ntiles;
+    a.per_xcd = p.per_xcd;
+    int rc;
+    EventPair* ev = p.timer.enabled ? p.timer.next(s) : nullptr;
+    if (ev) TW_HIP(hipEventRecord(ev->start, s));
+    if (p.order == 2) rc = launch_batch_ne<1>(p, a, s);
+    else if (p.order == 4) rc = launch_batch_ne<2>(p, a, s);
+    else rc = launch_batch_ne<4>(p, a, s);
+    if (rc) return rc;
+    TW_HIP(hipGetLastError());
+    if (ev) {
+        TW_HIP(hipEventRecord(ev->stop, s));
+        ev->kind = timing_kind(a.flags);
+    }
+    return TW_OK;
+}
+
+int batch_prepare(Plan& p, int what, int seg, int t0, hipStream_t s)
+{
+    long long bx = (p.max_field + 255) / 256;
+    if (bx > 1024) bx = 1024;
+    hipLaunchKernelGGL(k_batch_prepare, dim3((unsigned)bx, p.nshots), dim3(256), 0, s,
+                       p.d_desc, what, seg, t0);
+    TW_HIP(hipGetLastError());
+    return TW_OK;
+}
+
+}  // namespace tw
+
+using namespace tw;
+
+extern "C" {
+
+const char* tw_last_error(void) { return g_err.c_str(); }
+
+const char* tw_version(void) { return "torchwi_amd td2d 0.1 gfx950"; }
+
+int tw_td2d_forward(float* frd, float* virt, float* u1, float* u2, float* u3, const float* vpad,
+                    const float* w, int order, int dimx, int dimy, int nt, float h, float dt,
+                    float sx, float sy, float ry, void* stream)
+{
+    ShotDesc d;
+    int rc = single_desc(d, order, dimx, dimy, nt, h, dt, sx, sy, ry, true, vpad);
+    if (rc) return rc;
+    if (!frd || !virt || !u1 || !u2 || !u3 || !w) return fail(TW_EINVAL, "NULL buffer");
+    d.u[0] = u1; d.u[1] = u2; d.u[2] = u3;
+    d.frd = frd;
+    d.w = w;
+    d.virt = virt;
+    return run_forward_single(d, order, nt, M_FWD_VIRT, (hipStream_t)stream);
+}
+
+int tw_td2d_forward_only(float* frd, float* u1, float* u2, float* u3, const float* vpad,
+                         const float* w, int order, int dimx, int dimy, int nt, float h,
+                         float dt, float sx, float sy, float ry, void* stream)
+{
+    ShotDesc d;
+    int rc = single_desc(d, order, dimx, dimy, nt, h, dt, sx, sy, ry, true, vpad);
+    if (rc) return rc;
+    if (!frd || !u1 || !u2 || !u3 || !w) return fail(TW_EINVAL, "NULL buffer");
+    d.u[0] = u1; d.u[1] = u2; d.u[2] = u3;
+    d.frd = frd;
+    d.w = w;
+    return run_forward_single(d, order, nt, M_FWD, (hipStream_t)stream);
+}
+
+int tw_td2d_backward(float* grad, const float* virt, float* u1, float* u2, float* u3,
+                     const float* vpad, const float* resid, int order, int dimx, int dimy,
+                     int nt, float h, float dt, float ry, void* stream)
+{
+    ShotDesc d;
+    int rc = single_desc(d, order, dimx, dimy, nt, h, dt, 0.f, 0.f, ry, false, vpad);
+    if (rc) return rc;
+    if (!grad || !virt || !u1 || !u2 || !u3 || !resid) return fail(TW_EINVAL, "NULL buffer");
+    d.lam[0] = u1; d.lam[1] = u2; d.lam[2] = u3;
+    d.grad = grad;
+    d.virt = const_cast<float*>(virt);
+    d.resid = resid;
+    return run_adjoint_single(d, order, nt, M_ADJ, (hipStream_t)stream);
+}
+
+int tw_td2d_exa_forward(float* frd, float* exa, short* iexa, float* u1, float* u2, float* u3,
+                        const float* vpad, const float* w, int order, int dimx, int dimy, int nt,
+                        float h, float dt, float sx, float sy, float ry, void* stream)
+{
+    if (nt > 32767) return fail(TW_EINVAL, "EXA stores the step in int16: nt must be <= 32767");
+    ShotDesc d;
+    int rc = single_desc(d, order, dimx, dimy, nt, h, dt, sx, sy, ry, true, vpad);
+    if (rc) return rc;
+    if (!frd || !exa || !iexa || !u1 || !u2 || !u3 || !w) return fail(TW_EINVAL, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    d.u[0] = u1; d.u[1] = u2; d.u[2] = u3;
+    d.frd = frd;
+    d.w = w;
+    d.exa = exa;
+    d.iexa = iexa;
+    if ((rc = zero_f(exa, d.fstride, s))) return rc;
+    if ((rc = zero_i16(iexa, d.fstride, s))) return rc;
+    return run_forward_single(d, order, nt, M_EXA_FWD, s);
+}
+
+int tw_td2d_exa_backward(float* grad, const float* exa, const short* iexa, float* u1, float* u2,
+                         float* u3, const float* vpad, const float* resid, int order, int dimx,
+                         int dimy, int nt, float h, float dt, float ry, void* stream)
+{
+    if (nt > 32767) return fail(TW_EINVAL, "EXA stores the step in int16: nt must be <= 32767");
+    ShotDesc d;
+    int rc = single_desc(d, order, dimx, dimy, nt, h, dt, 0.f, 0.f, ry, false, vpad);
+    if (rc) return rc;
+    if (!grad || !exa || !iexa || !u1 || !u2 || !u3 || !resid) return fail(TW_EINVAL, "NULL buffer");
+    d.lam[0] = u1; d.lam[1] = u2; d.lam[2] = u3;
+    d.grad = grad;
+    d.exa = const_cast<float*>(exa);
+    d.iexa = const_cast<short*>(iexa);
+    d.resid = resid;
+    return run_adjoint_single(d, order, nt, M_EXA_ADJ, (hipStream_t)stream);
+}
+
+size_t tw_plan_workspace_bytes(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots)
+{
+    Plan p;
+    if (build_plan(spec, shots, nshots, p)) return 0;
+    return plan_bytes(p);
+}
+
+int tw_plan_create(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, void* workspace,
+                   size_t workspace_bytes, void* stream, void** plan)
+{
+    if (!plan) return fail(TW_EINVAL, "plan out-pointer is NULL");
+    *plan = nullptr;
+    Plan* p = new Plan();
+    int rc = build_plan(spec, shots, nshots, *p);
+    if (rc) { delete p; return rc; }
+    const size_t need = plan_bytes(*p);
+    if (!workspace || workspace_bytes < need) {
+        delete p;
+        return fail(TW_EINVAL, "workspace of %zu bytes is smaller than the %zu needed",
+                    workspace_bytes, need);
+    }
+    p->d_desc = (ShotDesc*)workspace;
+    p->d_tile_shot = (int*)((char*)workspace + align256(sizeof(ShotDesc) * p->nshots));
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(p->d_desc, p->host.data(), sizeof(ShotDesc) * p->nshots,
+                                  hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->d_tile_shot, p->tile_shot.data(), sizeof(int) * p->ntiles,
+                           hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        delete p;
+        return fail(TW_EHIP, "descriptor upload failed: %s", hipGetErrorString(e));
+    }
+    *plan = p;
+    return TW_OK;
+}
+
+int tw_plan_forward(void* plan, void* stream)
+{
+    if (!plan) return fail(TW_EINVAL, "NULL plan");
+    Plan& p = *(Plan*)plan;
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    // zero the whole forward ring: prepare(seg 0) zeroes two fields, the
+    // third is written completely by the first step.
+    if ((rc = batch_prepare(p, 0, 0, 0, s))) return rc;
+    StepArgs a;
+    std::memset(&a, 0, sizeof a);
+    for (int it = 0; it < p.nt; ++it) {
+        a.it = it;
+        a.ring = it;
+        a.adj = 0;
+        a.flags = M_FWD;
+        a.ck0 = a.ck1 = -1;
+        if (p.K == 0) {
+            a.flags = M_FWD_VIRT;
+            a.vslot = it;
+        } else {
+            if ((it + 2) % p.K == 0 && (it + 2) / p.K < p.nseg) a.ck0 = (it + 2) / p.K;
+            if ((it + 1) % p.K == 0 && (it + 1) / p.K < p.nseg) a.ck1 = (it + 1) / p.K;
+        }
+        if ((rc = launch_batch(p, a, s))) return rc;
+    }
+    return TW_OK;
+}
+
+int tw_plan_backward(void* plan, int accumulate, void* stream)
+{
+    if (!plan) return fail(TW_EINVAL, "NULL plan");
+    Plan& p = *(Plan*)plan;
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    for (int k = 0; k < p.nshots; ++k)
+        if (!p.host[k].resid) return fail(TW_EINVAL, "shot %d: NULL residual", k);
+    if (!accumulate && (rc = batch_prepare(p, 2, 0, 0, s))) return rc;
+    if ((rc = batch_prepare(p, 1, 0, 0, s))) return rc;
+    StepArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.ck0 = a.ck1 = -1;
+    auto adj_step = [&](int it, int slot) {
+        a.it = it;
+        a.ring = p.nt - 1 - it;
+        a.adj = 1;
+        a.flags = M_ADJ;
+        a.vslot = slot;
+        return launch_batch(p, a, s);
+    };
+    if (p.K == 0) {
+        for (int it = p.nt - 1; it >= 0; --it)
+            if ((rc = adj_step(it, it))) return rc;
+        return TW_OK;
+    }
+    for (int seg = p.nseg - 1; seg >= 0; --seg) {
+        const int t0 = seg * p.K;
+        const int t1 = (t0 + p.K < p.nt) ? t0 + p.K : p.nt;
+        if ((rc = batch_prepare(p, 0, seg, t0, s))) return rc;
+        for (int t = t0; t < t1; ++t) {
+            a.it = t;
+            a.ring = t;
+            a.adj = 0;
+            a.flags = M_RECOMP;
+            a.vslot = t - t0;
+            if ((rc = launch_batch(p, a, s))) return rc;
+        }
+        for (int it = t1 - 1; it >= t0; --it)
+            if ((rc = adj_step(it, it - t0))) return rc;
+    }
+    return TW_OK;
+}
+
+void tw_plan_destroy(void* plan) { delete (Plan*)plan; }
+
+int tw_plan_set_timing(void* plan, int enable)
+{
+    if (!plan) return fail(TW_EINVAL, "NULL plan");
+    Plan& p = *(Plan*)plan;
+    if (enable) {
+        int rc = p.timer.init(4096);
+        if (rc) return rc;
+    }
+    p.timer.enabled = enable != 0;
+    return TW_OK;
+}
+
+int tw_plan_timing(void* plan, int reset, double* ms, long long* count)
+{
+    if (!plan) return fail(TW_EINVAL, "NULL plan");
+    Plan& p = *(Plan*)plan;
+    p.timer.harvest_all();
+    for (int k = 0; k < TW_NKINDS; ++k) {
+        if (ms) ms[k] = p.timer.ms[k];
+        if (count) count[k] = p.timer.count[k];
+    }
+    if (reset) p.timer.reset();
+    return TW_OK;
+}
+
+}  // extern "C"
