@@ -45,7 +45,28 @@ def parse():
     ap.add_argument("--nt", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-launch HIP-event timing")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="first time a device-to-device copy of MB megabytes (achievable-bandwidth "
+                         "reference and PMC byte-counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(mb, reps=10):
+    """Achieved GB/s (read + write bytes) of a plain device copy of mb MiB."""
+    import torch
+    n = mb * (1 << 20) // 4
+    a = torch.rand(n, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"bytes_per_copy": 2 * n * 4, "avg_ms": round(ms, 4), "GBs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}
 
 
 def cpu_baseline():
@@ -94,6 +115,7 @@ def main():
     from torchwi_amd.batch import default_ckpt_interval
 
     dev = torch.device("cuda:0")
+    probe = copy_probe(args.copy_probe) if args.copy_probe else None
     c = W.config2(nshots=args.nshots, nt=args.nt)
     order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
     S = len(c["sx"])
@@ -166,6 +188,8 @@ def main():
         out["roofline"] = {"bound": "hbm", "kernel": "k_step_batch<4,%s>" % dom, "achieved": ach,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                            "traffic": None, "bytes_per_cell": b, "cells_per_launch": cells_launch}
+    if probe:
+        out["copy_probe"] = probe
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -108,14 +108,18 @@ typedef struct TwPlanSpec {
     float h, dt;      /* grid spacing, time step */
     int ckpt_every;   /* K > 0: checkpoint every K steps and recompute;
                          0: full virtual-source storage (nt slots) */
+    int tile_rows;    /* step-kernel tile height: 32 or 64, 0 = automatic */
 } TwPlanSpec;
 
 typedef struct TwShotSpec {
     int dimx, dimy;   /* padded grid of this shot */
-    int ld;           /* row stride in floats (>= dimx) */
+    int ld;           /* row stride in floats (>= dimx, multiple of 4); every
+                         field buffer below is 16-byte aligned */
     float sx, sy, ry; /* physical source / receiver-row positions */
     int rx0, nrec;    /* receiver window: interior columns [rx0, rx0+nrec) */
     const float *vel; /* (dimy, ld) padded model */
+    float *vm, *vc3;  /* (dimy, ld) derived-model work arrays, filled by the
+                         plan; shots sharing vel share them */
     const float *w;   /* (nt) wavelet */
     float *u;         /* forward ring: 3 fields of dimy*ld */
     float *lam;       /* adjoint ring: 3 fields of dimy*ld (may equal u when

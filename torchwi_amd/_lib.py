@@ -43,7 +43,8 @@ KINDS = ("fwd", "fwd_virt", "recomp", "adj")
 
 
 class TwPlanSpec(ctypes.Structure):
-    _fields_ = [("order", _ci), ("nt", _ci), ("h", _cf), ("dt", _cf), ("ckpt_every", _ci)]
+    _fields_ = [("order", _ci), ("nt", _ci), ("h", _cf), ("dt", _cf), ("ckpt_every", _ci),
+                ("tile_rows", _ci)]
 
 
 class TwShotSpec(ctypes.Structure):
@@ -51,7 +52,7 @@ class TwShotSpec(ctypes.Structure):
         ("dimx", _ci), ("dimy", _ci), ("ld", _ci),
         ("sx", _cf), ("sy", _cf), ("ry", _cf),
         ("rx0", _ci), ("nrec", _ci),
-        ("vel", _vp), ("w", _vp), ("u", _vp), ("lam", _vp), ("frd", _vp),
+        ("vel", _vp), ("vm", _vp), ("vc3", _vp), ("w", _vp), ("u", _vp), ("lam", _vp), ("frd", _vp),
         ("resid", _vp), ("virt", _vp), ("ckpt", _vp), ("grad", _vp),
     ]
 

@@ -42,7 +42,7 @@ def _as_list(x, name):
 class BatchState:
     """Device buffers and the C plan for one set of shots."""
 
-    def __init__(self, geom, sx, sy, ry, ckpt_every, device, rx0=None, nrec=None):
+    def __init__(self, geom, sx, sy, ry, ckpt_every, device, rx0=None, nrec=None, tile_rows=0):
         g = geom
         S = len(sx)
         self.S = S
@@ -54,6 +54,8 @@ class BatchState:
         dev = device
         zeros = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)  # noqa: E731
         self.vel = zeros(fshape)
+        self.vm = zeros(fshape)
+        self.vc3 = zeros(fshape)
         self.u = zeros(S, 3, f)
         self.lam = zeros(S, 3, f) if self.K > 0 else self.u
         nrec = g.nx if nrec is None else nrec
@@ -68,7 +70,8 @@ class BatchState:
         self.grad = zeros(S, f)
         self.w = g.w_dev
 
-        spec = _lib.TwPlanSpec(order=g.order, nt=g.nt, h=g.h, dt=g.dt, ckpt_every=self.K)
+        spec = _lib.TwPlanSpec(order=g.order, nt=g.nt, h=g.h, dt=g.dt, ckpt_every=self.K,
+                               tile_rows=int(tile_rows))
         shots = (_lib.TwShotSpec * S)()
         for s in range(S):
             q = shots[s]
@@ -76,6 +79,8 @@ class BatchState:
             q.sx, q.sy, q.ry = sx[s], sy[s], ry[s]
             q.rx0, q.nrec = rx0[s], nrec
             q.vel = self.vel.data_ptr()
+            q.vm = self.vm.data_ptr()
+            q.vc3 = self.vc3.data_ptr()
             q.w = self.w.data_ptr()
             q.u = self.u[s].data_ptr()
             q.lam = self.lam[s].data_ptr()
@@ -143,7 +148,7 @@ def state_bytes(geom, S, ckpt_every):
     K = int(ckpt_every)
     nseg = 0 if K == 0 else (geom.nt + K - 1) // K
     slots = geom.nt if K == 0 else K
-    fields = S * (3 + (3 if K else 0) + slots + 2 * nseg + 1) + 1
+    fields = S * (3 + (3 if K else 0) + slots + 2 * nseg + 1) + 3
     return 4 * (fields * f + 2 * S * geom.nt * geom.nx)
 
 
@@ -177,7 +182,7 @@ class Time2dBatch(torch.nn.Module):
     """
 
     def __init__(self, nx, ny, h, w, dt, order, device="cuda", ckpt_every="auto",
-                 hbm_fraction=0.8):
+                 hbm_fraction=0.8, tile_rows=0):
         super().__init__()
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -186,6 +191,7 @@ class Time2dBatch(torch.nn.Module):
         self.nt, self.nx, self.ny, self.order = self.g.nt, nx, ny, order
         self.ckpt_every = ckpt_every
         self.hbm_fraction = hbm_fraction
+        self.tile_rows = tile_rows
         self._state = None
         self._key = None
 
@@ -208,7 +214,7 @@ class Time2dBatch(torch.nn.Module):
             K = self.choose_ckpt(len(sx))
             self._state = None
             torch.cuda.empty_cache()
-            self._state = BatchState(self.g, sx, sy, ry, K, self.device)
+            self._state = BatchState(self.g, sx, sy, ry, K, self.device, tile_rows=self.tile_rows)
             self._key = key
         st = self._state
         self.load_model(vel, st)

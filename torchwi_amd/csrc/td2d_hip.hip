@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -53,6 +54,8 @@ struct ShotDesc {
     long long fstride;        // floats between consecutive fields of a ring
     float dt2, dtoh2, h2, hdt, cons;
     const float* vel;
+    const float* vm;          // (v*v)*dtoh2 per cell (batched path)
+    const float* vc3;         // cons/((v*v)*v) per cell (batched path)
     const float* w;
     float* u[3];              // forward ring
     float* lam[3];            // adjoint ring
@@ -187,17 +190,11 @@ __global__ __launch_bounds__(NTHR) void k_step_single(ShotDesc d, StepArgs a)
     step_tile<NE, FLAGS>(d, lt, a);
 }
 
-template <int NE, unsigned FLAGS>
-__global__ __launch_bounds__(NTHR) void k_step_batch(const ShotDesc* __restrict__ descs,
-                                                     const int* __restrict__ tile_shot,
-                                                     StepArgs a)
-{
-    const int t = logical_tile(blockIdx.x, a.per_xcd);
-    if (t >= a.ntiles) return;
-    const int s = tile_shot[t];
-    const ShotDesc& d = descs[s];
-    step_tile<NE, FLAGS>(d, t - d.tile0, a);
-}
+}  // namespace tw
+
+#include "td2d_vec.hpp"
+
+namespace tw {
 
 // Zero n floats (a fill kernel, not hipMemsetAsync, so the call can be
 // captured into a graph).
@@ -519,17 +516,44 @@ struct Timer {
     }
 };
 
+// A distinct padded model of a plan and its derived arrays.
+struct ModelRef {
+    const float* vel;
+    float* vm;
+    float* vc3;
+    int dimx, dimy, ld;
+    float dtoh2, cons;
+};
+
 struct Plan {
     Timer timer;
     int order = 0, nt = 0, K = 0, nseg = 0, nshots = 0, ntiles = 0, per_xcd = 0;
+    int ryv = 4;                      // rows per thread of the step kernel (tile height 16*ryv)
     long long max_field = 0;
     std::vector<ShotDesc> host;
-    std::vector<int> tile_shot;
+    std::vector<int2> tiles;          // (shot, by << 16 | bx), dealing order
+    std::vector<ModelRef> models;
     ShotDesc* d_desc = nullptr;
-    int* d_tile_shot = nullptr;
+    int2* d_tiles = nullptr;
 };
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Tile height for a set of grids: the smaller total of staged rows
+// (tiles_y * (tile_rows + 2*halo)), ties to the taller tile.
+int choose_ryv(const std::vector<int>& dimys, int ne)
+{
+    const int halo = ne < 2 ? 2 : ne;
+    long long best = -1;
+    int pick = 4;
+    for (int ryv : {4, 2}) {
+        const int ty = 16 * ryv;
+        long long cost = 0;
+        for (int dy : dimys) cost += (long long)((dy + ty - 1) / ty) * (ty + 2 * halo);
+        if (best < 0 || cost < best) { best = cost; pick = ryv; }
+    }
+    return pick;
+}
 
 int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan& p)
 {
@@ -543,11 +567,18 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
     p.nshots = nshots;
     const int ne = spec->order / 2;
     p.host.resize(nshots);
-    p.tile_shot.clear();
+    p.tiles.clear();
+    p.models.clear();
+    std::vector<int> dimys;
+    auto al16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
     for (int s = 0; s < nshots; ++s) {
         const TwShotSpec& q = shots[s];
         if ((rc = check_grid(spec->order, q.dimx, q.dimy, spec->nt, spec->h, spec->dt))) return rc;
-        if (q.ld < q.dimx) return fail(TW_EINVAL, "shot %d: ld %d < dimx %d", s, q.ld, q.dimx);
+        if (q.ld < q.dimx || q.ld % 4 != 0)
+            return fail(TW_EINVAL, "shot %d: row stride ld=%d must be >= dimx=%d and a multiple of 4",
+                        s, q.ld, q.dimx);
+        if (q.dimx >= (1 << 16) * VTX / 2 || q.dimy >= (1 << 15) * 16)
+            return fail(TW_EINVAL, "shot %d: grid %dx%d too large for the tile table", s, q.dimx, q.dimy);
         Cells c = truncate_positions(q.sx, q.sy, q.ry, spec->h);
         if ((rc = check_src(c, spec->order, q.dimx, q.dimy))) return rc;
         if ((rc = check_rec(c, spec->order, q.dimy))) return rc;
@@ -555,20 +586,19 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
         if (q.rx0 < 0 || q.nrec <= 0 || q.rx0 + q.nrec > nx)
             return fail(TW_EINVAL, "shot %d: receiver window [%d,%d) outside [0,%d)", s, q.rx0,
                         q.rx0 + q.nrec, nx);
-        if (!q.vel || !q.w || !q.u || !q.lam || !q.frd || !q.grad || !q.virt)
+        if (!q.vel || !q.vm || !q.vc3 || !q.w || !q.u || !q.lam || !q.frd || !q.grad || !q.virt)
             return fail(TW_EINVAL, "shot %d: NULL buffer", s);
         if (p.K > 0 && !q.ckpt) return fail(TW_EINVAL, "shot %d: NULL checkpoint buffer", s);
         if (p.K > 0 && q.lam == q.u)
             return fail(TW_EINVAL, "shot %d: checkpointing needs separate forward and adjoint rings", s);
+        if (!al16(q.vel) || !al16(q.vm) || !al16(q.vc3) || !al16(q.u) || !al16(q.lam) ||
+            !al16(q.virt) || !al16(q.grad) || (q.ckpt && !al16(q.ckpt)))
+            return fail(TW_EINVAL, "shot %d: field buffers must be 16-byte aligned", s);
         ShotDesc& d = p.host[s];
         std::memset(&d, 0, sizeof d);
         d.dimx = q.dimx;
         d.dimy = q.dimy;
         d.ld = q.ld;
-        int ty;
-        grid_tiles(q.dimx, q.dimy, d.tiles_x, ty);
-        d.tile0 = (int)p.tile_shot.size();
-        for (int k = 0; k < d.tiles_x * ty; ++k) p.tile_shot.push_back(s);
         d.rrow = c.iry + ne;
         d.rx0 = q.rx0 + ne;
         d.nrec = q.nrec;
@@ -577,6 +607,8 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
         if (d.fstride > p.max_field) p.max_field = d.fstride;
         fill_consts(d, spec->h, spec->dt);
         d.vel = q.vel;
+        d.vm = q.vm;
+        d.vc3 = q.vc3;
         d.w = q.w;
         for (int k = 0; k < 3; ++k) {
             d.u[k] = q.u + k * d.fstride;
@@ -587,22 +619,53 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
         d.virt = q.virt;
         d.ckpt = q.ckpt;
         d.grad = q.grad;
+        bool seen = false;
+        for (const ModelRef& m : p.models) {
+            if (m.vel == q.vel) {
+                if (m.vm != q.vm || m.vc3 != q.vc3 || m.dimx != q.dimx || m.dimy != q.dimy || m.ld != q.ld)
+                    return fail(TW_EINVAL, "shot %d: shots sharing a model must share its grid and derived arrays", s);
+                seen = true;
+            }
+        }
+        if (!seen) {
+            p.models.push_back({q.vel, q.vm, q.vc3, q.dimx, q.dimy, q.ld, d.dtoh2, d.cons});
+            dimys.push_back(q.dimy);
+        }
     }
-    p.ntiles = (int)p.tile_shot.size();
+    if (spec->tile_rows == 32 || spec->tile_rows == 64) p.ryv = spec->tile_rows / 16;
+    else if (spec->tile_rows == 0) p.ryv = choose_ryv(dimys, ne);
+    else return fail(TW_EINVAL, "tile_rows must be 0, 32 or 64 (got %d)", spec->tile_rows);
+    const int ty = 16 * p.ryv;
+    // Deal tiles model by model; within a model spatial-tile-major and
+    // shot-minor, so consecutive workgroups of an XCD share model rows.
+    for (const ModelRef& m : p.models) {
+        std::vector<int> group;
+        for (int s = 0; s < nshots; ++s)
+            if (p.host[s].vel == m.vel) group.push_back(s);
+        const int ntx = (m.dimx + VTX - 1) / VTX, nty = (m.dimy + ty - 1) / ty;
+        for (int by = 0; by < nty; ++by)
+            for (int bx = 0; bx < ntx; ++bx)
+                for (int s : group) p.tiles.push_back(make_int2(s, (by << 16) | bx));
+    }
+    p.ntiles = (int)p.tiles.size();
     p.per_xcd = (p.ntiles + NXCD - 1) / NXCD;
     return TW_OK;
 }
 
 size_t plan_bytes(const Plan& p)
 {
-    return align256(sizeof(ShotDesc) * p.nshots) + align256(sizeof(int) * p.ntiles);
+    return align256(sizeof(ShotDesc) * p.nshots) + align256(sizeof(int2) * p.ntiles);
 }
 
 template <int NE, unsigned FLAGS>
 void launch_batch_nf(const Plan& p, const StepArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_step_batch<NE, FLAGS>), dim3(p.per_xcd * NXCD), dim3(NTHR), 0, s,
-                       p.d_desc, p.d_tile_shot, a);
+    if (p.ryv == 4)
+        hipLaunchKernelGGL((k_vstep<NE, FLAGS, 4>), dim3(p.per_xcd * NXCD), dim3(VTHR), 0, s,
+                           p.d_desc, p.d_tiles, a);
+    else
+        hipLaunchKernelGGL((k_vstep<NE, FLAGS, 2>), dim3(p.per_xcd * NXCD), dim3(VTHR), 0, s,
+                           p.d_desc, p.d_tiles, a);
 }
 
 template <int NE>
@@ -645,6 +708,20 @@ int launch_batch(Plan& p, StepArgs a, hipStream_t s)
     if (ev) {
         TW_HIP(hipEventRecord(ev->stop, s));
         ev->kind = timing_kind(a.flags);
+    }
+    return TW_OK;
+}
+
+// Recompute the derived arrays of every model of the plan.
+int prep_models(const Plan& p, hipStream_t s)
+{
+    for (const ModelRef& m : p.models) {
+        long long n = (long long)m.dimy * m.ld;
+        long long bx = (n + 255) / 256;
+        if (bx > 2048) bx = 2048;
+        hipLaunchKernelGGL(k_model_prep, dim3((unsigned)bx), dim3(256), 0, s, m.vel, m.vm, m.vc3,
+                           m.dimx, m.dimy, m.ld, m.dtoh2, m.cons);
+        TW_HIP(hipGetLastError());
     }
     return TW_OK;
 }
@@ -772,12 +849,12 @@ int tw_plan_create(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, 
                     workspace_bytes, need);
     }
     p->d_desc = (ShotDesc*)workspace;
-    p->d_tile_shot = (int*)((char*)workspace + align256(sizeof(ShotDesc) * p->nshots));
+    p->d_tiles = (int2*)((char*)workspace + align256(sizeof(ShotDesc) * p->nshots));
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = hipMemcpyAsync(p->d_desc, p->host.data(), sizeof(ShotDesc) * p->nshots,
                                   hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(p->d_tile_shot, p->tile_shot.data(), sizeof(int) * p->ntiles,
+        e = hipMemcpyAsync(p->d_tiles, p->tiles.data(), sizeof(int2) * p->ntiles,
                            hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
@@ -794,8 +871,9 @@ int tw_plan_forward(void* plan, void* stream)
     Plan& p = *(Plan*)plan;
     hipStream_t s = (hipStream_t)stream;
     int rc;
-    // zero the whole forward ring: prepare(seg 0) zeroes two fields, the
-    // third is written completely by the first step.
+    if ((rc = prep_models(p, s))) return rc;
+    // zero the forward ring: prepare(seg 0) zeroes two fields, the third is
+    // written completely by the first step.
     if ((rc = batch_prepare(p, 0, 0, 0, s))) return rc;
     StepArgs a;
     std::memset(&a, 0, sizeof a);
@@ -825,6 +903,7 @@ int tw_plan_backward(void* plan, int accumulate, void* stream)
     int rc;
     for (int k = 0; k < p.nshots; ++k)
         if (!p.host[k].resid) return fail(TW_EINVAL, "shot %d: NULL residual", k);
+    if ((rc = prep_models(p, s))) return rc;
     if (!accumulate && (rc = batch_prepare(p, 2, 0, 0, s))) return rc;
     if ((rc = batch_prepare(p, 1, 0, 0, s))) return rc;
     StepArgs a;
