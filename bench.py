@@ -45,6 +45,10 @@ def parse():
     ap.add_argument("--nt", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-launch HIP-event timing")
+    ap.add_argument("--graph-steps", type=int, default=100,
+                    help="time steps per captured hipGraph block (0: one direct launch per step)")
+    ap.add_argument("--compare", type=int, default=1,
+                    help="also time the checkpointed form and the plain one-shot-per-launch form")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="first time a device-to-device copy of MB megabytes (achievable-bandwidth "
                          "reference and PMC byte-counter calibration)")
@@ -103,6 +107,88 @@ def cpu_baseline():
                       "forward with full storage + adjoint gradient; median of 5 runs (%.3f s each)" % t}
 
 
+def run_batched(c, K, steps, warmup, timing, graph_steps, dev):
+    """Time `steps` misfit gradients of the batched engine; returns stats."""
+    import torch
+    from torchwi_amd import Time2dBatch
+    order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
+    S = len(c["sx"])
+    model = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K,
+                        graph_steps=graph_steps)
+    vel = torch.from_numpy(c["vel"]).to(dev)
+    rng = np.random.default_rng(7)
+    vtrue = torch.from_numpy(c["vel"] * (1.0 + 0.02 * rng.standard_normal(c["vel"].shape)).astype(np.float32)).to(dev)
+    d_obs = model.modeling(vtrue, c["sx"], c["sy"], c["ry"]).clone()
+    st = model.plan
+    mg = st.models[0]
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        model.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
+    torch.cuda.synchronize()
+    if timing:
+        st.set_timing(True)
+        st.timing(reset=True)
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        model.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    cells_step = 2.0 * S * mg.dimx * mg.dimy * nt
+    r = {"K": st.K, "S": S, "dimx": mg.dimx, "dimy": mg.dimy, "elapsed": elapsed, "lat": lat,
+         "value": cells_step * steps / elapsed / 1e9, "shots_per_s": S * steps / elapsed}
+    if timing:
+        r["timing"] = st.timing()
+        st.set_timing(False)
+    del model, st
+    torch.cuda.empty_cache()
+    return r
+
+
+def run_plain(c, nshots, dev):
+    """The plain form: one shot at a time through the drop-in Time2d module
+    (full storage, one launch per step), as the reference's loop calls it
+    (inv_torch.py:128-137).  Returns gradient-shots/s."""
+    import torch
+    import torchwi_amd as torchwi
+    order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
+    model = torchwi.Time2d(nx, ny, h, torch.from_numpy(c["w"]), dt, order, "cuda")
+    vel = torch.nn.Parameter(torch.from_numpy(c["vel"]).to(dev))
+    true = torch.zeros(nt, nx, device=dev)
+
+    def one(k):
+        frd = model(vel, torch.tensor([c["sx"][k]]), torch.tensor([c["sy"][k]]), torch.tensor([c["ry"][k]]))
+        loss = 0.5 * torch.sum((frd - true) ** 2)
+        loss.backward()
+
+    one(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(nshots):
+        one(k % len(c["sx"]))
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    del model
+    torch.cuda.empty_cache()
+    return {"gradient_shots_per_s": round(nshots / el, 3), "shots_timed": nshots,
+            "s_per_shot": round(el / nshots, 4)}
+
+
+def load_traffic(kernel_key):
+    """HBM bytes per launch of a step kernel at this workload, from the
+    calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs kept in profiles/."""
+    path = os.path.join(ROOT, "profiles", "traffic_k_vstep_config2.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"].get(kernel_key)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -111,50 +197,17 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
-    from torchwi_amd import Time2dBatch, workloads as W
-    from torchwi_amd.batch import default_ckpt_interval
+    from torchwi_amd import workloads as W
+    from torchwi_amd.shots import default_ckpt_interval
 
     dev = torch.device("cuda:0")
     probe = copy_probe(args.copy_probe) if args.copy_probe else None
     c = W.config2(nshots=args.nshots, nt=args.nt)
     order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
-    S = len(c["sx"])
-    if args.mode == "full":
-        K = 0
-    elif args.mode == "ckpt":
-        K = args.ckpt_every or default_ckpt_interval(nt)
-    else:
-        K = "auto"
-    model = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K)
-    vel = torch.from_numpy(c["vel"]).to(dev)
-    rng = np.random.default_rng(7)
-    vtrue = torch.from_numpy(c["vel"] * (1.0 + 0.02 * rng.standard_normal(c["vel"].shape)).astype(np.float32)).to(dev)
-    d_obs = model.modeling(vtrue, c["sx"], c["sy"], c["ry"]).clone()
-    st = model._state
-    g = model.g
-    torch.cuda.synchronize()
-
-    for _ in range(args.warmup):
-        model.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
-    torch.cuda.synchronize()
-    if not args.no_timing:
-        st.set_timing(True)
-        st.timing(reset=True)
-
-    lat = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        loss, grad = model.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
-        torch.cuda.synchronize()
-        lat.append((time.perf_counter() - ts) * 1e3)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-
-    cells_step = 2.0 * S * g.dimx * g.dimy * nt
-    value = cells_step * args.steps / elapsed / 1e9
-    kstate = st.K
+    kck = args.ckpt_every or default_ckpt_interval(nt)
+    K = {"full": 0, "ckpt": kck, "auto": "auto"}[args.mode]
+    r = run_batched(c, K, args.steps, args.warmup, not args.no_timing, args.graph_steps, dev)
+    S, kstate, value, elapsed, lat = r["S"], r["K"], r["value"], r["elapsed"], r["lat"]
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -163,18 +216,18 @@ def main():
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "batched misfit gradient (forward + adjoint), %d shots, %dx%d model "
                                "(padded %dx%d), h=%g m, dt=%g s, nt=%d, order %d, %s"
-                               % (S, ny, nx, g.dimy, g.dimx, h, dt, nt, order,
+                               % (S, ny, nx, r["dimy"], r["dimx"], h, dt, nt, order,
                                   "full virtual-source storage" if kstate == 0 else "checkpoint every %d steps + recompute" % kstate),
-                   "shots": S, "nz": ny, "nx": nx, "nt": nt, "order": order, "ckpt_every": kstate},
+                   "shots": S, "nz": ny, "nx": nx, "nt": nt, "order": order, "ckpt_every": kstate,
+                   "graph_steps": args.graph_steps},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
-        "gradient_shots_per_s": round(S * args.steps / elapsed, 3),
+        "gradient_shots_per_s": round(r["shots_per_s"], 3),
         "effective_hbm_frac": round(value * 1e9 * (BYTES_GRAD_STEP / 2.0) / (HBM_PEAK_GBS * 1e9), 4),
     }
     if not args.no_timing:
-        tm = st.timing()
-        st.set_timing(False)
+        tm = r["timing"]
         per = {}
-        cells_launch = S * g.dimx * g.dimy
+        cells_launch = S * r["dimx"] * r["dimy"]
         for kind, b in (("fwd", BYTES_FWD), ("fwd_virt", BYTES_FWD_VIRT), ("recomp", BYTES_FWD_VIRT), ("adj", BYTES_ADJ)):
             ms, n = tm[kind]
             if n:
@@ -185,9 +238,24 @@ def main():
         dom = max(per, key=lambda k: per[k]["avg_us"] * per[k]["launches"])
         b = {"fwd": BYTES_FWD, "fwd_virt": BYTES_FWD_VIRT, "recomp": BYTES_FWD_VIRT, "adj": BYTES_ADJ}[dom]
         ach = per[dom]["achieved_GBs"]
-        out["roofline"] = {"bound": "hbm", "kernel": "k_step_batch<4,%s>" % dom, "achieved": ach,
+        tr = load_traffic(dom) if (nt, S, order) == (4000, 32, 8) else None
+        out["roofline"] = {"bound": "hbm", "kernel": "k_vstep<4,%s>" % dom, "achieved": ach,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                           "traffic": None, "bytes_per_cell": b, "cells_per_launch": cells_launch}
+                           "traffic": tr, "bytes_per_cell": b, "cells_per_launch": cells_launch}
+    if args.compare:
+        if kstate == 0:
+            rc = run_batched(c, kck, 1, 0, False, args.graph_steps, dev)
+            out["checkpointed"] = {"ckpt_every": rc["K"], "value": round(rc["value"], 3),
+                                   "ms_per_step": round(rc["elapsed"] * 1e3, 3),
+                                   "gradient_shots_per_s": round(rc["shots_per_s"], 3),
+                                   "effective_hbm_frac": round(rc["value"] * 1e9 * (BYTES_GRAD_STEP / 2.0) / (HBM_PEAK_GBS * 1e9), 4)}
+            ck_rate = rc["shots_per_s"]
+        else:
+            ck_rate = r["shots_per_s"]
+        pl = run_plain(c, 2, dev)
+        out["plain_one_shot"] = pl
+        out["speedup_vs_plain"] = {"best": round(r["shots_per_s"] / pl["gradient_shots_per_s"], 2),
+                                   "checkpointed": round(ck_rate / pl["gradient_shots_per_s"], 2)}
     if probe:
         out["copy_probe"] = probe
     if not args.no_cpu_baseline:

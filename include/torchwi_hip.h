@@ -109,6 +109,10 @@ typedef struct TwPlanSpec {
     int ckpt_every;   /* K > 0: checkpoint every K steps and recompute;
                          0: full virtual-source storage (nt slots) */
     int tile_rows;    /* step-kernel tile height: 32 or 64, 0 = automatic */
+    int graph_steps;  /* > 0: time steps are launched as replays of hipGraphs
+                         captured once per plan (this many steps per forward /
+                         full-storage adjoint block; one graph per checkpoint
+                         segment); 0: one direct launch per step */
 } TwPlanSpec;
 
 typedef struct TwShotSpec {
@@ -160,8 +164,9 @@ void tw_plan_destroy(void *plan);
 #define TW_KIND_ADJ 3      /* adjoint + imaging condition */
 #define TW_NKINDS 4
 
-/* Enable (1) or disable (0) HIP-event timing of every step-kernel launch of
- * this plan (an event pair around each launch, on the launch stream). */
+/* Enable (1) or disable (0) HIP-event timing of this plan's step kernels: an
+ * event pair around each direct launch, or around each graph replay (then
+ * counted as the replay's number of step kernels), on the launch stream. */
 int tw_plan_set_timing(void *plan, int enable);
 
 /* Accumulated event time (ms) and launch count per kind (arrays of

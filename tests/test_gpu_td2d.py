@@ -157,24 +157,24 @@ def test_batch_matches_single_and_checkpoint_matches_full(order):
     d_obs.copy_(0.8 * frd_full)
     loss_f, g_full = full.misfit_gradient(vel, sx, sy, ry, d_obs)
     g_full = g_full.clone()
-    per_shot_full = full._state.grad.clone()
+    per_shot_full = [full.plan.grad_of(s).clone() for s in range(4)]
     for K in (7, 64, 1000):
         ck = Time2dBatch(c["nx"], c["ny"], c["h"], w, c["dt"], order, DEV, ckpt_every=K)
         loss_c, g_ck = ck.misfit_gradient(vel, sx, sy, ry, d_obs)
-        assert torch.equal(ck._state.frd, frd_full)
+        assert torch.equal(ck.plan.frd_all(), frd_full)
         assert torch.equal(g_ck, g_full), f"K={K}"
         assert float(loss_c) == float(loss_f)
     # each shot of the batch equals the one-shot oracle
     vpad = O.pad_vel(c["vel"], order)
     args = (order, c["dimx"], c["dimy"], c["h"], c["dt"])
-    g = full.g
+    dimx = c["dimx"]
     for s in range(4):
         frd_o, virt_o = O.forward(vpad, c["w"], *args, sx[s], sy[s], ry[s])
         assert np.array_equal(frd_full[s].cpu().numpy(), frd_o)
         resid = frd_o - 0.8 * frd_o
         resid = (frd_full[s] - d_obs[s]).cpu().numpy()
         grad_o = O.backward(virt_o, vpad, resid, *args, ry[s])
-        got = per_shot_full[s].view(g.dimy, g.ld)[:, :g.dimx].cpu().numpy().ravel()
+        got = per_shot_full[s][:, :dimx].cpu().numpy().ravel()
         assert np.array_equal(got, grad_o)
 
 
@@ -243,14 +243,79 @@ def test_batch_kernel_shapes_bitexact(order, nx, ny, tile_rows):
     frd = m.modeling(vel, sx, sy, ry).clone()
     d_obs = torch.zeros_like(frd)
     m.misfit_gradient(vel, sx, sy, ry, d_obs)
-    g = m.g
     vpad = O.pad_vel(c["vel"], order)
     args = (order, c["dimx"], c["dimy"], c["h"], c["dt"])
+    dimx = c["dimx"]
     for s in range(3):
         frd_o, virt_o = O.forward(vpad, c["w"], *args, sx[s], sy[s], ry[s])
         assert np.array_equal(frd[s].cpu().numpy(), frd_o), f"shot {s} gather"
         grad_o = O.backward(virt_o, vpad, frd_o, *args, ry[s])
-        got = m._state.grad[s].view(g.dimy, g.ld)[:, :g.dimx].cpu().numpy().ravel()
+        got = m.plan.grad_of(s)[:, :dimx].cpu().numpy().ravel()
         assert np.array_equal(got, grad_o), f"shot {s} gradient"
-        pad = m._state.grad[s].view(g.dimy, g.ld)[:, g.dimx:]
+        pad = m.plan.grad_of(s)[:, dimx:]
         assert not torch.any(pad != 0), "row padding must stay zero"
+
+
+@pytest.mark.parametrize("K", [0, 13])
+def test_graph_replay_matches_direct_launches(K):
+    """hipGraph-captured step blocks (partial last block / segment included)
+    give the same bits as one direct launch per step."""
+    from torchwi_amd import Time2dBatch
+    order = 8
+    c = small_case(order, nx=120, ny=50, nt=230, seed=21)
+    sx = [(i + 0.5) * c["h"] for i in (10, 60, 110)]
+    sy = [2.5 * c["h"]] * 3
+    ry = [3.5 * c["h"]] * 3
+    vel = torch.from_numpy(c["vel"]).to(DEV)
+    w = torch.from_numpy(c["w"])
+    out = []
+    for gs in (0, 7, 100):
+        print(f"case K={K} graph_steps={gs}", flush=True)
+        m = Time2dBatch(c["nx"], c["ny"], c["h"], w, c["dt"], order, DEV, ckpt_every=K, graph_steps=gs)
+        frd = m.modeling(vel, sx, sy, ry).clone()
+        loss, g = m.misfit_gradient(vel, sx, sy, ry, 0.5 * frd)
+        out.append((frd, g.clone(), float(loss)))
+    for frd, g, loss in out[1:]:
+        assert torch.equal(frd, out[0][0])
+        assert torch.equal(g, out[0][1])
+        assert loss == out[0][2]
+
+
+def test_windows_match_oracle_on_cropped_models():
+    """Shots on uneven aperture windows of one model, in one plan: each gather
+    and gradient equals the oracle on the cropped sub-model (the reference's
+    Time2d on that sub-model), and the full gradient is their windowed sum."""
+    from torchwi_amd import Time2dWindows
+    order, nx, ny, nt, h, dt = 8, 260, 40, 150, 10.0, 1e-3
+    rng = np.random.default_rng(31)
+    vel = (1500 + 2000 * rng.random((nx, ny))).astype(np.float32)
+    t = np.arange(nt) * dt
+    w = ((1 - 2 * (np.pi * 15 * (t - 0.05)) ** 2) * np.exp(-(np.pi * 15 * (t - 0.05)) ** 2)).astype(np.float32)
+    wins = []
+    for (x0, x1) in ((0, 97), (40, 201), (150, 260), (61, 62 + 70)):
+        nxw = x1 - x0
+        wins.append(dict(x0=x0, x1=x1, sx=(nxw // 2 + 0.5) * h, sy=2.5 * h, ry=2.5 * h,
+                         rx0=5, nrec=nxw - 10))
+    m = Time2dWindows(nx, ny, h, torch.from_numpy(w), dt, order, wins, DEV, ckpt_every=0)
+    velt = torch.from_numpy(vel).to(DEV)
+    frds = [f.clone() for f in m.modeling(velt)]
+    d_obs = [0.7 * f for f in frds]
+    loss, g = m.misfit_gradient(velt, d_obs)
+    want = np.zeros((nx, ny), np.float64)
+    for s, q in enumerate(wins):
+        sub = vel[q["x0"]:q["x1"], :]
+        nxw = q["x1"] - q["x0"]
+        vpad = O.pad_vel(sub, order)
+        a = (order, nxw + order, ny + order, h, dt)
+        frd_o, virt_o = O.forward(vpad, w, *a, q["sx"], q["sy"], q["ry"])
+        win = frd_o[:, q["rx0"]:q["rx0"] + q["nrec"]]
+        assert np.array_equal(frds[s].cpu().numpy(), win), f"window {s}"
+        resid = np.zeros_like(frd_o)
+        resid[:, q["rx0"]:q["rx0"] + q["nrec"]] = win - 0.7 * win
+        # the residual the GPU formed is frd - 0.7 frd in float32
+        resid[:, q["rx0"]:q["rx0"] + q["nrec"]] = (frds[s] - d_obs[s]).cpu().numpy()
+        g_o = O.crop_grad(O.backward(virt_o, vpad, resid, *a, q["ry"]), order, nxw, ny)
+        got = m.plan.grad_of(s)[order // 2:order // 2 + ny, order // 2:order // 2 + nxw].cpu().numpy().T
+        assert np.array_equal(got, g_o), f"window {s} gradient"
+        want[q["x0"]:q["x1"], :] += g_o
+    assert rel_l2(g.cpu().numpy(), want) < 1e-6

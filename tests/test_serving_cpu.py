@@ -1,0 +1,41 @@
+"""Host-side checks of the serving loop's admission arithmetic (no GPU)."""
+import torch
+
+from torchwi_amd.serving import Request, chunk_plan, io_bytes, make_stream
+from torchwi_amd.shots import default_ckpt_interval, plan_bytes
+
+
+def _req(kind, nx=200, ny=80, nt=400, S=10):
+    return Request(kind=kind, vel=torch.full((nx, ny), 2000.0), w=torch.zeros(nt), sx=[15.0] * S,
+                   sy=[25.0] * S, ry=[25.0] * S, h=10.0, dt=1e-3,
+                   d_obs=torch.zeros(1) if kind == "gradient" else None)
+
+
+def test_gradient_uses_full_storage_when_it_fits_the_cap():
+    r = _req("gradient")
+    full = plan_bytes([(r.nx, r.ny, 8)], [dict(model=0, nrec=r.nx)] * r.nshots, r.nt, 0)
+    per, K, b = chunk_plan(r, full)
+    assert (per, K, b) == (r.nshots, 0, full)
+
+
+def test_gradient_chunks_with_checkpoints_under_a_small_cap():
+    r = _req("gradient", nt=900)
+    K = default_ckpt_interval(r.nt)
+    three = plan_bytes([(r.nx, r.ny, 8)], [dict(model=0, nrec=r.nx)] * 3, r.nt, K)
+    per, k2, b = chunk_plan(r, three + 1)
+    assert (per, k2, b) == (3, K, three)
+
+
+def test_chunking_depends_on_the_request_only():
+    r = _req("forward", S=7)
+    assert chunk_plan(r, 10 ** 9) == chunk_plan(_req("forward", S=7), 10 ** 9)
+    assert io_bytes(r) == 4 * (r.nx * r.ny + r.nt) + 4 * 7 * r.nt * r.nx
+
+
+def test_stream_generator_is_seeded():
+    a = make_stream(5, 2.0, seed=1, grids=((32, 64),), nt_range=(50, 60), shots_range=(1, 3), device="cpu")
+    b = make_stream(5, 2.0, seed=1, grids=((32, 64),), nt_range=(50, 60), shots_range=(1, 3), device="cpu")
+    assert [(x.kind, x.nt, x.nshots, x.arrival) for x in a] == [(x.kind, x.nt, x.nshots, x.arrival) for x in b]
+    assert all(torch.equal(x.vel, y.vel) for x, y in zip(a, b))
+    arr = [x.arrival for x in a]
+    assert arr == sorted(arr) and arr[0] > 0

@@ -3,35 +3,28 @@
 The reference computes one shot per call and spreads shots over processes
 (examples/time/inv_torch.py:123-143 with io/DistributedDataloader.py:7-12):
 every call runs its own time loop, so one GPU advances one small grid per
-launch.  ``Time2dBatch`` instead advances every shot of a batch by one time
-step per launch (the C-ABI plan in include/torchwi_hip.h), keeps all state
-device-resident, and replaces full virtual-source storage by checkpoints of
+launch.  The classes here advance every shot of a batch by one time step
+per launch (torchwi_amd.shots.ShotPlan over the C-ABI plan), keep all state
+device-resident, and replace full virtual-source storage by checkpoints of
 the wavefield every K steps plus recomputation when storage would not fit.
 
 Each shot's result is computed exactly as a single ``Time2d`` call computes
-it (same kernels, same per-cell arithmetic); the batch gradient is the sum
-of the per-shot gradients in shot order.
-"""
-import ctypes
-import math
+it (same per-cell arithmetic); a batch gradient is the sum of the per-shot
+gradients in shot order.
 
+* ``Time2dBatch``   -- S shots on one (nx, ny) model, receivers on every
+                       interior column (the reference's acquisition).
+* ``Time2dWindows`` -- S shots, each on its own aperture window cropped
+                       from a large model, with its own receiver window.
+"""
 import torch
 
-from . import _lib
+from .shots import GRAPH_STEPS, ShotPlan, default_ckpt_interval, plan_bytes
 
-_ALIGN = 32  # row stride alignment in floats (128 B)
-
-
-def round_up(n, m):
-    return (n + m - 1) // m * m
+__all__ = ["Time2dBatch", "Time2dWindows", "default_ckpt_interval"]
 
 
-def default_ckpt_interval(nt):
-    """About sqrt(nt) steps per checkpoint segment."""
-    return max(2, int(math.ceil(math.sqrt(nt))))
-
-
-def _as_list(x, name):
+def _as_list(x):
     if isinstance(x, torch.Tensor):
         return [float(v) for v in x.detach().reshape(-1).cpu().tolist()]
     if isinstance(x, (int, float)):
@@ -39,117 +32,20 @@ def _as_list(x, name):
     return [float(v) for v in x]
 
 
-class BatchState:
-    """Device buffers and the C plan for one set of shots."""
-
-    def __init__(self, geom, sx, sy, ry, ckpt_every, device, rx0=None, nrec=None, tile_rows=0):
-        g = geom
-        S = len(sx)
-        self.S = S
-        self.K = int(ckpt_every)
-        self.nt = g.nt
-        fshape = (g.dimy, g.ld)
-        f = g.dimy * g.ld
-        self.fstride = f
-        dev = device
-        zeros = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)  # noqa: E731
-        self.vel = zeros(fshape)
-        self.vm = zeros(fshape)
-        self.vc3 = zeros(fshape)
-        self.u = zeros(S, 3, f)
-        self.lam = zeros(S, 3, f) if self.K > 0 else self.u
-        nrec = g.nx if nrec is None else nrec
-        rx0 = [0] * S if rx0 is None else rx0
-        self.nrec = nrec
-        self.frd = zeros(S, g.nt, nrec)
-        self.resid = zeros(S, g.nt, nrec)
-        nslots = g.nt if self.K == 0 else self.K
-        self.nseg = 0 if self.K == 0 else (g.nt + self.K - 1) // self.K
-        self.virt = torch.empty(S, nslots, f, dtype=torch.float32, device=dev)
-        self.ckpt = torch.empty(S, 2 * self.nseg, f, dtype=torch.float32, device=dev) if self.K else None
-        self.grad = zeros(S, f)
-        self.w = g.w_dev
-
-        spec = _lib.TwPlanSpec(order=g.order, nt=g.nt, h=g.h, dt=g.dt, ckpt_every=self.K,
-                               tile_rows=int(tile_rows))
-        shots = (_lib.TwShotSpec * S)()
-        for s in range(S):
-            q = shots[s]
-            q.dimx, q.dimy, q.ld = g.dimx, g.dimy, g.ld
-            q.sx, q.sy, q.ry = sx[s], sy[s], ry[s]
-            q.rx0, q.nrec = rx0[s], nrec
-            q.vel = self.vel.data_ptr()
-            q.vm = self.vm.data_ptr()
-            q.vc3 = self.vc3.data_ptr()
-            q.w = self.w.data_ptr()
-            q.u = self.u[s].data_ptr()
-            q.lam = self.lam[s].data_ptr()
-            q.frd = self.frd[s].data_ptr()
-            q.resid = self.resid[s].data_ptr()
-            q.virt = self.virt[s].data_ptr()
-            q.ckpt = self.ckpt[s].data_ptr() if self.K else None
-            q.grad = self.grad[s].data_ptr()
-        L = _lib.lib()
-        nbytes = L.tw_plan_workspace_bytes(ctypes.byref(spec), shots, S)
-        if nbytes == 0:
-            raise RuntimeError("torchwi_amd: invalid shot batch: %s" % _lib.last_error())
-        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-        handle = ctypes.c_void_p()
-        rc = L.tw_plan_create(ctypes.byref(spec), shots, S, _lib.ptr(self.workspace), nbytes,
-                              _lib.stream_of(self.workspace), ctypes.byref(handle))
-        _lib.check(rc, "tw_plan_create")
-        self.plan = handle
-
-    def __del__(self):
-        try:
-            if getattr(self, "plan", None):
-                _lib.lib().tw_plan_destroy(self.plan)
-                self.plan = None
-        except Exception:
-            pass
-
-    def forward(self):
-        _lib.check(_lib.lib().tw_plan_forward(self.plan, _lib.stream_of(self.u)), "tw_plan_forward")
-
-    def set_timing(self, enable):
-        _lib.check(_lib.lib().tw_plan_set_timing(self.plan, int(bool(enable))), "tw_plan_set_timing")
-
-    def timing(self, reset=False):
-        """{kind: (total ms, launches)} of HIP-event-timed step kernels."""
-        n = len(_lib.KINDS)
-        ms = (ctypes.c_double * n)()
-        cnt = (ctypes.c_longlong * n)()
-        _lib.check(_lib.lib().tw_plan_timing(self.plan, int(bool(reset)), ms, cnt), "tw_plan_timing")
-        return {k: (ms[i], cnt[i]) for i, k in enumerate(_lib.KINDS)}
-
-    def backward(self, accumulate=False):
-        _lib.check(_lib.lib().tw_plan_backward(self.plan, int(bool(accumulate)),
-                                               _lib.stream_of(self.u)), "tw_plan_backward")
+def _wavelet(w):
+    w = torch.as_tensor(w)
+    if w.dtype != torch.float32:
+        raise RuntimeError("w: expected scalar type Float but found %s" % w.dtype)
+    return w.contiguous()
 
 
-class _Geom:
-    def __init__(self, nx, ny, h, w, dt, order, device):
-        if order not in (2, 4, 8):
-            raise ValueError("Wrong order: use 2, 4, and 8")
-        self.nx, self.ny, self.order, self.ne = nx, ny, order, order // 2
-        self.dimx, self.dimy = nx + order, ny + order
-        self.ld = round_up(self.dimx, _ALIGN)
-        self.h, self.dt = float(h), float(dt)
-        w = torch.as_tensor(w)
-        if w.dtype != torch.float32:
-            raise RuntimeError("w: expected scalar type Float but found %s" % w.dtype)
-        self.nt = int(w.numel())
-        self.w_dev = w.to(device).contiguous()
-
-
-def state_bytes(geom, S, ckpt_every):
-    """Device bytes a BatchState of S shots needs."""
-    f = geom.dimy * geom.ld
-    K = int(ckpt_every)
-    nseg = 0 if K == 0 else (geom.nt + K - 1) // K
-    slots = geom.nt if K == 0 else K
-    fields = S * (3 + (3 if K else 0) + slots + 2 * nseg + 1) + 3
-    return 4 * (fields * f + 2 * S * geom.nt * geom.nx)
+def _choose_ckpt(ckpt_every, need_full, nt, device, hbm_fraction, held=0):
+    if ckpt_every != "auto":
+        return int(ckpt_every)
+    free, _ = torch.cuda.mem_get_info(device)
+    if need_full <= hbm_fraction * (free + held):
+        return 0
+    return default_ckpt_interval(nt)
 
 
 class _BatchOperator(torch.autograd.Function):
@@ -157,17 +53,17 @@ class _BatchOperator(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vel, args):
         sx, sy, ry, m = args
-        st = m.prepare(vel, sx, sy, ry)
-        st.forward()
-        ctx.m, ctx.st = m, st
-        return st.frd
+        p = m.prepare(vel, sx, sy, ry)
+        p.forward()
+        ctx.m, ctx.p = m, p
+        return p.frd_all()
 
     @staticmethod
     def backward(ctx, grad_output):
-        m, st = ctx.m, ctx.st
-        st.resid.copy_(grad_output)
-        st.backward()
-        return m.reduce_grad(st), None
+        m, p = ctx.m, ctx.p
+        p.resid_all().copy_(grad_output)
+        p.backward()
+        return m.reduce_grad(p), None
 
 
 class Time2dBatch(torch.nn.Module):
@@ -179,66 +75,63 @@ class Time2dBatch(torch.nn.Module):
     ``ckpt_every``: 0 = full virtual-source storage, K > 0 = checkpoint
     every K steps and recompute, "auto" = full storage when it fits in
     ``hbm_fraction`` of free device memory, else K = ceil(sqrt(nt)).
+    The returned gathers are the plan's buffer: the next forward overwrites
+    them (as the reference's module buffer, TimeFunction.py:128).
     """
 
     def __init__(self, nx, ny, h, w, dt, order, device="cuda", ckpt_every="auto",
-                 hbm_fraction=0.8, tile_rows=0):
+                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS):
         super().__init__()
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise NotImplementedError("Time2dBatch runs on the GPU only")
-        self.g = _Geom(nx, ny, h, w, dt, order, self.device)
-        self.nt, self.nx, self.ny, self.order = self.g.nt, nx, ny, order
+        self.graph_steps = graph_steps
+        if order not in (2, 4, 8):
+            raise ValueError("Wrong order: use 2, 4, and 8")
+        self.nx, self.ny, self.order, self.ne = nx, ny, order, order // 2
+        self.h, self.dt = float(h), float(dt)
+        self.w = _wavelet(w)
+        self.nt = int(self.w.numel())
         self.ckpt_every = ckpt_every
         self.hbm_fraction = hbm_fraction
         self.tile_rows = tile_rows
-        self._state = None
+        self._plan = None
         self._key = None
 
-    def choose_ckpt(self, S):
-        if self.ckpt_every != "auto":
-            return int(self.ckpt_every)
-        free, _ = torch.cuda.mem_get_info(self.device)
-        if self._state is not None:
-            free += state_bytes(self.g, self._state.S, self._state.K)
-        if state_bytes(self.g, S, 0) <= self.hbm_fraction * free:
-            return 0
-        return default_ckpt_interval(self.nt)
+    def _shots(self, sx, sy, ry):
+        return [dict(model=0, sx=a, sy=b, ry=c, rx0=0, nrec=self.nx, w=self.w) for a, b, c in zip(sx, sy, ry)]
 
     def prepare(self, vel, sx, sy, ry):
-        sx, sy, ry = _as_list(sx, "sx"), _as_list(sy, "sy"), _as_list(ry, "ry")
+        sx, sy, ry = _as_list(sx), _as_list(sy), _as_list(ry)
         if not (len(sx) == len(sy) == len(ry)) or len(sx) == 0:
             raise ValueError("sx, sy, ry must have the same positive length")
         key = (tuple(sx), tuple(sy), tuple(ry))
         if self._key != key:
-            K = self.choose_ckpt(len(sx))
-            self._state = None
+            shots = self._shots(sx, sy, ry)
+            models = [(self.nx, self.ny, self.order)]
+            held = plan_bytes(models, self._plan.shots, self.nt, self._plan.K) if self._plan else 0
+            K = _choose_ckpt(self.ckpt_every, plan_bytes(models, shots, self.nt, 0), self.nt,
+                             self.device, self.hbm_fraction, held)
+            self._plan = None
             torch.cuda.empty_cache()
-            self._state = BatchState(self.g, sx, sy, ry, K, self.device, tile_rows=self.tile_rows)
+            self._plan = ShotPlan(self.order, self.nt, self.h, self.dt, [(self.nx, self.ny)], shots, K,
+                                  self.device, tile_rows=self.tile_rows, graph_steps=self.graph_steps)
             self._key = key
-        st = self._state
-        self.load_model(vel, st)
-        return st
+        self._plan.models[0].load(vel)
+        return self._plan
 
-    def load_model(self, vel, st):
-        g = self.g
-        if tuple(vel.shape) != (g.nx, g.ny):
-            raise ValueError("vel must be (nx, ny) = (%d, %d), got %s" % (g.nx, g.ny, tuple(vel.shape)))
-        with torch.no_grad():
-            vp = torch.nn.functional.pad(
-                vel.detach().to(self.device, torch.float32).transpose(0, 1).reshape(1, 1, g.ny, g.nx),
-                (g.ne, g.ne, g.ne, g.ne), mode="replicate").reshape(g.dimy, g.dimx)
-            st.vel[:, :g.dimx].copy_(vp)
+    @property
+    def plan(self):
+        return self._plan
 
-    def reduce_grad(self, st):
+    def reduce_grad(self, p):
         """Sum the per-shot padded gradients in shot order, crop to the model
         and transpose to (nx, ny) (TimeFunction.py:141-145)."""
-        g = self.g
-        tot = st.grad[0].clone()
-        for s in range(1, st.S):
-            tot.add_(st.grad[s])
-        tot = tot.view(g.dimy, g.ld)[g.ne:g.ne + g.ny, g.ne:g.ne + g.nx]
-        return tot.transpose(0, 1)
+        ne = self.ne
+        tot = p.grad_of(0).clone()
+        for s in range(1, p.S):
+            tot.add_(p.grad_of(s))
+        return tot[ne:ne + self.ny, ne:ne + self.nx].transpose(0, 1)
 
     def forward(self, vel, sx, sy, ry):
         return _BatchOperator.apply(vel, (sx, sy, ry, self))
@@ -246,17 +139,84 @@ class Time2dBatch(torch.nn.Module):
     @torch.no_grad()
     def misfit_gradient(self, vel, sx, sy, ry, d_obs):
         """One FWI gradient: forward all shots, residual frd - d_obs, L2 loss
-        0.5*||frd - d_obs||^2, adjoint, summed gradient.  Returns (loss,
-        grad (nx, ny))."""
-        st = self.prepare(vel, sx, sy, ry)
-        st.forward()
-        torch.sub(st.frd, d_obs, out=st.resid)
-        loss = 0.5 * torch.sum(st.resid * st.resid)
-        st.backward()
-        return loss, self.reduce_grad(st)
+        0.5*||frd - d_obs||^2 (inv_torch.py:134), adjoint, summed gradient.
+        Returns (loss, grad (nx, ny))."""
+        p = self.prepare(vel, sx, sy, ry)
+        p.forward()
+        r = p.resid_all()
+        torch.sub(p.frd_all(), d_obs, out=r)
+        rv = r.view(-1)
+        loss = 0.5 * torch.dot(rv, rv)
+        p.backward()
+        return loss, self.reduce_grad(p)
 
     @torch.no_grad()
     def modeling(self, vel, sx, sy, ry):
-        st = self.prepare(vel, sx, sy, ry)
-        st.forward()
-        return st.frd
+        p = self.prepare(vel, sx, sy, ry)
+        p.forward()
+        return p.frd_all()
+
+
+class Time2dWindows(torch.nn.Module):
+    """Shots on per-shot aperture windows of one large model, in one plan.
+
+    windows: list of dicts {x0, x1, sx, sy, ry, rx0, nrec} in cells of the
+    full model: the shot's sub-model is columns [x0, x1) of vel (all depths),
+    edge-replicated like any Time2d model; sx, sy, ry are physical positions
+    inside the window (relative to its left edge); receivers are window
+    columns [rx0, rx0+nrec).  A shot's results equal Time2d on its cropped
+    sub-model.  Gradients of all windows are summed into the full model's
+    (nx, ny) gradient in shot order.
+    """
+
+    def __init__(self, nx, ny, h, w, dt, order, windows, device="cuda", ckpt_every="auto",
+                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS):
+        super().__init__()
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise NotImplementedError("Time2dWindows runs on the GPU only")
+        self.nx, self.ny, self.order, self.ne = nx, ny, order, order // 2
+        self.h, self.dt = float(h), float(dt)
+        self.w = _wavelet(w)
+        self.nt = int(self.w.numel())
+        self.windows = [dict(q) for q in windows]
+        for q in self.windows:
+            if not (0 <= q["x0"] < q["x1"] <= nx):
+                raise ValueError("window [%d, %d) outside [0, %d)" % (q["x0"], q["x1"], nx))
+        models = [(q["x1"] - q["x0"], ny) for q in self.windows]
+        shots = [dict(model=i, sx=q["sx"], sy=q["sy"], ry=q["ry"], rx0=q["rx0"], nrec=q["nrec"],
+                      w=q.get("w", self.w)) for i, q in enumerate(self.windows)]
+        K = _choose_ckpt(ckpt_every, plan_bytes([(a, b, order) for a, b in models], shots, self.nt, 0),
+                         self.nt, self.device, hbm_fraction)
+        self.plan = ShotPlan(order, self.nt, h, dt, models, shots, K, self.device, tile_rows=tile_rows,
+                             graph_steps=graph_steps)
+
+    def load(self, vel):
+        for i, q in enumerate(self.windows):
+            self.plan.models[i].load(vel[q["x0"]:q["x1"], :])
+
+    @torch.no_grad()
+    def modeling(self, vel):
+        self.load(vel)
+        self.plan.forward()
+        return [self.plan.frd_of(s) for s in range(self.plan.S)]
+
+    @torch.no_grad()
+    def misfit_gradient(self, vel, d_obs):
+        """d_obs: list of (nt, nrec_s) gathers.  Returns (loss, grad (nx, ny))."""
+        p = self.plan
+        self.load(vel)
+        p.forward()
+        loss = torch.zeros((), dtype=torch.float32, device=self.device)
+        for s in range(p.S):
+            r = p.resid_of(s)
+            torch.sub(p.frd_of(s), d_obs[s], out=r)
+            rv = r.view(-1)
+            loss += 0.5 * torch.dot(rv, rv)
+        p.backward()
+        ne = self.ne
+        grad = torch.zeros(self.ny, self.nx, dtype=torch.float32, device=self.device)
+        for s, q in enumerate(self.windows):
+            nxw = q["x1"] - q["x0"]
+            grad[:, q["x0"]:q["x1"]] += p.grad_of(s)[ne:ne + self.ny, ne:ne + nxw]
+        return loss, grad.transpose(0, 1)

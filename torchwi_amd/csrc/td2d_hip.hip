@@ -68,6 +68,18 @@ struct ShotDesc {
     short* iexa;
 };
 
+// How a launch finds its time step.  PH_DIRECT: it/ring/vslot/ck0/ck1 are
+// given.  The other phases derive them on the device from the node index j
+// and a control word *ctl (a block or segment index kept in device memory),
+// so that one captured graph of steps can be replayed for the whole loop.
+enum : int {
+    PH_DIRECT = 0,
+    PH_FWD_BLK = 1,     // forward step t = ctl*B + j
+    PH_BWD_BLK = 2,     // full-storage adjoint step it = nt-1 - (ctl*B + j)
+    PH_RECOMP_SEG = 3,  // recompute step t = seg*K + j of segment seg = ctl
+    PH_ADJ_SEG = 4,     // adjoint step it = min(nt, seg*K+K) - 1 - j of segment seg
+};
+
 struct StepArgs {
     int it;          // time index of this step
     int ring;        // rotation: u_prev = ring%3, u_cur = (ring+1)%3, u_new = (ring+2)%3
@@ -77,7 +89,55 @@ struct StepArgs {
     int adj;         // 1: step the adjoint ring
     int ntiles;      // tiles in the launch
     int per_xcd;     // ceil(ntiles / NXCD)
+    int phase, j, B; // step resolution (see PH_*)
+    int nt, K, nseg;
+    const int* ctl;
 };
+
+// Resolve the step of a launch; false when the node lies past the end of the
+// time loop (a partial last block or segment), in which case the whole
+// launch does nothing.
+__device__ __forceinline__ bool resolve_step(StepArgs& a)
+{
+    if (a.phase == PH_DIRECT) return true;
+    const int c = *a.ctl;
+    if (a.phase == PH_FWD_BLK) {
+        const int t = c * a.B + a.j;
+        if (t >= a.nt) return false;
+        a.it = t;
+        a.ring = t;
+        a.vslot = a.K == 0 ? t : 0;
+        a.ck0 = a.ck1 = -1;
+        if (a.K > 0) {
+            if ((t + 2) % a.K == 0 && (t + 2) / a.K < a.nseg) a.ck0 = (t + 2) / a.K;
+            if ((t + 1) % a.K == 0 && (t + 1) / a.K < a.nseg) a.ck1 = (t + 1) / a.K;
+        }
+    } else if (a.phase == PH_BWD_BLK) {
+        const int s = c * a.B + a.j;
+        if (s >= a.nt) return false;
+        a.it = a.nt - 1 - s;
+        a.ring = s;
+        a.vslot = a.it;
+        a.ck0 = a.ck1 = -1;
+    } else if (a.phase == PH_RECOMP_SEG) {
+        const int t = c * a.K + a.j;
+        if (t >= a.nt) return false;
+        a.it = t;
+        a.ring = t;
+        a.vslot = a.j;
+        a.ck0 = a.ck1 = -1;
+    } else {
+        const int t0 = c * a.K;
+        const int t1 = t0 + a.K < a.nt ? t0 + a.K : a.nt;
+        const int it = t1 - 1 - a.j;
+        if (it < t0) return false;
+        a.it = it;
+        a.ring = a.nt - 1 - it;
+        a.vslot = it - t0;
+        a.ck0 = a.ck1 = -1;
+    }
+    return true;
+}
 
 // Map a hardware workgroup id to a logical tile so that each XCD receives a
 // contiguous band of tiles (shared halos stay in that XCD's L2).
@@ -215,8 +275,13 @@ __global__ void k_zero_i16(short* __restrict__ p, long long n)
 // Per shot (blockIdx.y), prepare the forward ring for a segment starting at
 // step t0: u_prev <- ckpt[seg][0], u_cur <- ckpt[seg][1] (zero for seg 0);
 // or (what = 1) zero the whole adjoint ring; or (what = 2) zero grad.
-__global__ void k_batch_prepare(const ShotDesc* __restrict__ descs, int what, int seg, int t0)
+__global__ void k_batch_prepare(const ShotDesc* __restrict__ descs, int what, int seg, int t0,
+                                const int* __restrict__ ctl, int K)
 {
+    if (ctl) {  // segment index from the control word (graph replay)
+        seg = *ctl;
+        t0 = seg * K;
+    }
     const ShotDesc& d = descs[blockIdx.y];
     const long long n = (long long)d.dimy * d.ld;
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -238,6 +303,12 @@ __global__ void k_batch_prepare(const ShotDesc* __restrict__ descs, int what, in
     } else {
         for (long long i = i0; i < n; i += stride) d.grad[i] = 0.f;
     }
+}
+
+// Control word update: op 0 sets *ctl = v, op 1 adds v.
+__global__ void k_ctl(int* ctl, int op, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *ctl = op == 0 ? v : *ctl + v;
 }
 
 // ------------------------------------------------------------------------
@@ -460,6 +531,7 @@ int run_adjoint_single(ShotDesc& d, int order, int nt, unsigned flags, hipStream
 struct EventPair {
     hipEvent_t start = nullptr, stop = nullptr;
     int kind = -1;
+    int steps = 0;   // step kernels between start and stop
 };
 
 struct Timer {
@@ -486,7 +558,7 @@ struct Timer {
             if (hipEventSynchronize(e.stop) == hipSuccess &&
                 hipEventElapsedTime(&t, e.start, e.stop) == hipSuccess) {
                 ms[e.kind] += t;
-                count[e.kind] += 1;
+                count[e.kind] += e.steps;
             }
             e.kind = -1;
         }
@@ -525,8 +597,25 @@ struct ModelRef {
     float dtoh2, cons;
 };
 
+// A captured, instantiated graph.
+struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    void release()
+    {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+        x = nullptr;
+        g = nullptr;
+    }
+};
+
 struct Plan {
     Timer timer;
+    int graph_steps = 0;              // steps per captured block (0: direct launches)
+    hipStream_t cap = nullptr;        // private stream graphs are captured on
+    Graph g_fwd, g_bwd, g_recomp, g_adj;
+    int* d_ctl = nullptr;             // device control words (block / segment index)
     int order = 0, nt = 0, K = 0, nseg = 0, nshots = 0, ntiles = 0, per_xcd = 0;
     int ryv = 4;                      // rows per thread of the step kernel (tile height 16*ryv)
     long long max_field = 0;
@@ -535,6 +624,14 @@ struct Plan {
     std::vector<ModelRef> models;
     ShotDesc* d_desc = nullptr;
     int2* d_tiles = nullptr;
+    ~Plan()
+    {
+        g_fwd.release();
+        g_bwd.release();
+        g_recomp.release();
+        g_adj.release();
+        if (cap) (void)hipStreamDestroy(cap);
+    }
 };
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
@@ -563,6 +660,8 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
     p.nt = spec->nt;
     p.K = spec->ckpt_every;
     if (p.K < 0) return fail(TW_EINVAL, "ckpt_every must be >= 0");
+    if (spec->graph_steps < 0 || spec->graph_steps > 4096)
+        return fail(TW_EINVAL, "graph_steps must be in [0, 4096] (got %d)", spec->graph_steps);
     p.nseg = p.K > 0 ? (p.nt + p.K - 1) / p.K : 0;
     p.nshots = nshots;
     const int ne = spec->order / 2;
@@ -654,7 +753,7 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
 
 size_t plan_bytes(const Plan& p)
 {
-    return align256(sizeof(ShotDesc) * p.nshots) + align256(sizeof(int2) * p.ntiles);
+    return align256(sizeof(ShotDesc) * p.nshots) + align256(sizeof(int2) * p.ntiles) + 256;
 }
 
 template <int NE, unsigned FLAGS>
@@ -693,12 +792,17 @@ int timing_kind(unsigned flags)
     }
 }
 
-int launch_batch(Plan& p, StepArgs a, hipStream_t s)
+// Launch one step kernel.  timed: bracket it with an event pair (direct
+// launches only; inside a capture the caller times the whole graph).
+int launch_batch(Plan& p, StepArgs a, hipStream_t s, bool timed = true)
 {
     a.ntiles = p.ntiles;
     a.per_xcd = p.per_xcd;
+    a.nt = p.nt;
+    a.K = p.K;
+    a.nseg = p.nseg;
     int rc;
-    EventPair* ev = p.timer.enabled ? p.timer.next(s) : nullptr;
+    EventPair* ev = (timed && p.timer.enabled) ? p.timer.next(s) : nullptr;
     if (ev) TW_HIP(hipEventRecord(ev->start, s));
     if (p.order == 2) rc = launch_batch_ne<1>(p, a, s);
     else if (p.order == 4) rc = launch_batch_ne<2>(p, a, s);
@@ -708,6 +812,82 @@ int launch_batch(Plan& p, StepArgs a, hipStream_t s)
     if (ev) {
         TW_HIP(hipEventRecord(ev->stop, s));
         ev->kind = timing_kind(a.flags);
+        ev->steps = 1;
+    }
+    return TW_OK;
+}
+
+int set_ctl(const Plan& p, int idx, int op, int v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, s, p.d_ctl + idx, op, v);
+    TW_HIP(hipGetLastError());
+    return TW_OK;
+}
+
+// Capture the four step-block graphs of a plan (once, on its private stream).
+int batch_prepare_ctl(Plan& p, hipStream_t s);
+
+int capture(Plan& p, Graph& gr, int which)
+{
+    if (gr.x) return TW_OK;
+    if (!p.cap) TW_HIP(hipStreamCreateWithFlags(&p.cap, hipStreamNonBlocking));
+    TW_HIP(hipStreamBeginCapture(p.cap, hipStreamCaptureModeThreadLocal));
+    int rc = TW_OK;
+    StepArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.ctl = p.d_ctl;
+    a.B = p.graph_steps;
+    if (which == 0) {         // forward block: B steps, then ctl0 += 1
+        a.phase = PH_FWD_BLK;
+        a.flags = p.K == 0 ? M_FWD_VIRT : M_FWD;
+        for (int j = 0; j < p.graph_steps && !rc; ++j) { a.j = j; rc = launch_batch(p, a, p.cap, false); }
+        if (!rc) rc = set_ctl(p, 0, 1, 1, p.cap);
+    } else if (which == 1) {  // full-storage adjoint block: B steps, then ctl0 += 1
+        a.phase = PH_BWD_BLK;
+        a.adj = 1;
+        a.flags = M_ADJ;
+        for (int j = 0; j < p.graph_steps && !rc; ++j) { a.j = j; rc = launch_batch(p, a, p.cap, false); }
+        if (!rc) rc = set_ctl(p, 0, 1, 1, p.cap);
+    } else if (which == 2) {  // segment recompute: restore from ctl1, K steps
+        rc = batch_prepare_ctl(p, p.cap);
+        a.phase = PH_RECOMP_SEG;
+        a.ctl = p.d_ctl + 1;
+        a.flags = M_RECOMP;
+        for (int j = 0; j < p.K && !rc; ++j) { a.j = j; rc = launch_batch(p, a, p.cap, false); }
+    } else {                  // segment adjoint: K steps, then ctl1 -= 1
+        a.phase = PH_ADJ_SEG;
+        a.ctl = p.d_ctl + 1;
+        a.adj = 1;
+        a.flags = M_ADJ;
+        for (int j = 0; j < p.K && !rc; ++j) { a.j = j; rc = launch_batch(p, a, p.cap, false); }
+        if (!rc) rc = set_ctl(p, 1, 1, -1, p.cap);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(p.cap, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(TW_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+    gr.g = g;
+    e = hipGraphInstantiate(&gr.x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        gr.release();
+        return fail(TW_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    return TW_OK;
+}
+
+// Replay a graph on the caller's stream, timed as `steps` kernels of `kind`.
+int replay(Plan& p, Graph& gr, int kind, int steps, hipStream_t s)
+{
+    EventPair* ev = p.timer.enabled ? p.timer.next(s) : nullptr;
+    if (ev) TW_HIP(hipEventRecord(ev->start, s));
+    TW_HIP(hipGraphLaunch(gr.x, s));
+    if (ev) {
+        TW_HIP(hipEventRecord(ev->stop, s));
+        ev->kind = kind;
+        ev->steps = steps;
     }
     return TW_OK;
 }
@@ -731,7 +911,18 @@ int batch_prepare(Plan& p, int what, int seg, int t0, hipStream_t s)
     long long bx = (p.max_field + 255) / 256;
     if (bx > 1024) bx = 1024;
     hipLaunchKernelGGL(k_batch_prepare, dim3((unsigned)bx, p.nshots), dim3(256), 0, s,
-                       p.d_desc, what, seg, t0);
+                       p.d_desc, what, seg, t0, (const int*)nullptr, p.K);
+    TW_HIP(hipGetLastError());
+    return TW_OK;
+}
+
+// Restore the forward ring for the segment named by control word 1.
+int batch_prepare_ctl(Plan& p, hipStream_t s)
+{
+    long long bx = (p.max_field + 255) / 256;
+    if (bx > 1024) bx = 1024;
+    hipLaunchKernelGGL(k_batch_prepare, dim3((unsigned)bx, p.nshots), dim3(256), 0, s,
+                       p.d_desc, 0, 0, 0, (const int*)(p.d_ctl + 1), p.K);
     TW_HIP(hipGetLastError());
     return TW_OK;
 }
@@ -850,6 +1041,9 @@ int tw_plan_create(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, 
     }
     p->d_desc = (ShotDesc*)workspace;
     p->d_tiles = (int2*)((char*)workspace + align256(sizeof(ShotDesc) * p->nshots));
+    p->d_ctl = (int*)((char*)workspace + align256(sizeof(ShotDesc) * p->nshots) +
+                      align256(sizeof(int2) * p->ntiles));
+    p->graph_steps = spec->graph_steps;
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = hipMemcpyAsync(p->d_desc, p->host.data(), sizeof(ShotDesc) * p->nshots,
                                   hipMemcpyHostToDevice, s);
@@ -875,6 +1069,15 @@ int tw_plan_forward(void* plan, void* stream)
     // zero the forward ring: prepare(seg 0) zeroes two fields, the third is
     // written completely by the first step.
     if ((rc = batch_prepare(p, 0, 0, 0, s))) return rc;
+    const int kind = p.K == 0 ? TW_KIND_FWD_VIRT : TW_KIND_FWD;
+    if (p.graph_steps > 0) {
+        if ((rc = capture(p, p.g_fwd, 0))) return rc;
+        if ((rc = set_ctl(p, 0, 0, 0, s))) return rc;
+        const int B = p.graph_steps;
+        for (int t0 = 0; t0 < p.nt; t0 += B)
+            if ((rc = replay(p, p.g_fwd, kind, (p.nt - t0 < B ? p.nt - t0 : B), s))) return rc;
+        return TW_OK;
+    }
     StepArgs a;
     std::memset(&a, 0, sizeof a);
     for (int it = 0; it < p.nt; ++it) {
@@ -906,6 +1109,26 @@ int tw_plan_backward(void* plan, int accumulate, void* stream)
     if ((rc = prep_models(p, s))) return rc;
     if (!accumulate && (rc = batch_prepare(p, 2, 0, 0, s))) return rc;
     if ((rc = batch_prepare(p, 1, 0, 0, s))) return rc;
+    if (p.graph_steps > 0) {
+        if (p.K == 0) {
+            if ((rc = capture(p, p.g_bwd, 1))) return rc;
+            if ((rc = set_ctl(p, 0, 0, 0, s))) return rc;
+            const int B = p.graph_steps;
+            for (int t0 = 0; t0 < p.nt; t0 += B)
+                if ((rc = replay(p, p.g_bwd, TW_KIND_ADJ, (p.nt - t0 < B ? p.nt - t0 : B), s))) return rc;
+            return TW_OK;
+        }
+        if ((rc = capture(p, p.g_recomp, 2))) return rc;
+        if ((rc = capture(p, p.g_adj, 3))) return rc;
+        if ((rc = set_ctl(p, 1, 0, p.nseg - 1, s))) return rc;
+        for (int seg = p.nseg - 1; seg >= 0; --seg) {
+            const int t0 = seg * p.K;
+            const int n = (t0 + p.K < p.nt ? p.K : p.nt - t0);
+            if ((rc = replay(p, p.g_recomp, TW_KIND_RECOMP, n, s))) return rc;
+            if ((rc = replay(p, p.g_adj, TW_KIND_ADJ, n, s))) return rc;
+        }
+        return TW_OK;
+    }
     StepArgs a;
     std::memset(&a, 0, sizeof a);
     a.ck0 = a.ck1 = -1;

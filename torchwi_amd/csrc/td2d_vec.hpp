@@ -65,7 +65,7 @@ __global__ __launch_bounds__(VTHR) void k_vstep(const ShotDesc* __restrict__ des
     __shared__ float4 T4[H * W4];
 
     const int L = logical_tile(blockIdx.x, a.per_xcd);
-    if (L >= a.ntiles) return;
+    if (L >= a.ntiles || !resolve_step(a)) return;
     const int2 e = tiles[L];
     const ShotDesc& d = descs[e.x];
     const int x0 = (e.y & 0xffff) * VTX, y0 = (e.y >> 16) * TYV;

@@ -1,0 +1,215 @@
+"""Device-resident shot plans: the host side of the batched C-ABI plan.
+
+A ``ShotPlan`` holds every device buffer of a set of shots that march in
+lockstep (one launch per time step for all of them) and the C plan built
+over them.  Shots may live on different grids: each shot names one of the
+plan's models (a padded velocity grid with its derived arrays), so a plan
+can carry many shots of one model (the reference's shot loop,
+examples/time/inv_torch.py:123-143) or one cropped sub-model per shot
+(aperture windows).  All per-shot buffers of one kind share one arena
+tensor, so the C plan sees plain pointers and the caching allocator sees a
+handful of large allocations.
+"""
+import ctypes
+import math
+
+import torch
+
+from . import _lib
+
+ALIGN = 32  # row stride alignment in floats (128 B)
+GRAPH_STEPS = 100  # time steps per captured hipGraph block (0: direct launches)
+
+
+def round_up(n, m):
+    return (n + m - 1) // m * m
+
+
+def default_ckpt_interval(nt):
+    """About sqrt(nt) steps per checkpoint segment."""
+    return max(2, int(math.ceil(math.sqrt(nt))))
+
+
+class ModelGrid:
+    """A padded model of an (nx, ny) interior: vel and the derived m, c3."""
+
+    def __init__(self, nx, ny, order, device):
+        self.nx, self.ny, self.order, self.ne = nx, ny, order, order // 2
+        self.dimx, self.dimy = nx + order, ny + order
+        self.ld = round_up(self.dimx, ALIGN)
+        z = lambda: torch.zeros(self.dimy, self.ld, dtype=torch.float32, device=device)  # noqa: E731
+        self.vel, self.vm, self.vc3 = z(), z(), z()
+
+    @property
+    def fstride(self):
+        return self.dimy * self.ld
+
+    def load(self, vel):
+        """Edge-replicate vel (nx, ny), depth fastest, into the padded model,
+        as TimeOperator.pad_vel does (TimeOperator.py:55-58)."""
+        if tuple(vel.shape) != (self.nx, self.ny):
+            raise ValueError("vel must be (nx, ny) = (%d, %d), got %s" % (self.nx, self.ny, tuple(vel.shape)))
+        ne = self.ne
+        with torch.no_grad():
+            vp = torch.nn.functional.pad(
+                vel.detach().to(self.vel.device, torch.float32).transpose(0, 1).reshape(1, 1, self.ny, self.nx),
+                (ne, ne, ne, ne), mode="replicate").reshape(self.dimy, self.dimx)
+            self.vel[:, :self.dimx].copy_(vp)
+
+
+def plan_bytes(models, shots, nt, ckpt_every):
+    """Device bytes of a ShotPlan (models: list of (nx, ny, order);
+    shots: list of dicts with 'model' and 'nrec')."""
+    K = int(ckpt_every)
+    nseg = 0 if K == 0 else (nt + K - 1) // K
+    slots = nt if K == 0 else K
+    fs = []
+    tot = 0
+    for nx, ny, order in models:
+        f = (ny + order) * round_up(nx + order, ALIGN)
+        fs.append(f)
+        tot += 3 * f
+    for q in shots:
+        f = fs[q["model"]]
+        tot += f * (3 + (3 if K else 0) + slots + 2 * nseg + 1) + 2 * nt * q["nrec"]
+    return 4 * tot
+
+
+class ShotPlan:
+    """Buffers and C plan of shots marching in lockstep.
+
+    models: list of (nx, ny) interiors (all of one FD order).
+    shots:  list of dicts {model, sx, sy, ry, rx0, nrec, w}; sx, sy, ry are
+            physical positions within the shot's model, [rx0, rx0+nrec) its
+            receiver columns (interior coordinates), w its (nt,) float32
+            wavelet (a tensor; shots may share one).
+    ckpt_every: 0 = full virtual-source storage, K > 0 = checkpoints.
+    graph_steps: time steps per replayed hipGraph block (0: one direct launch
+            per step).
+    """
+
+    def __init__(self, order, nt, h, dt, models, shots, ckpt_every, device, tile_rows=0,
+                 graph_steps=GRAPH_STEPS):
+        self.order, self.nt, self.h, self.dt = order, int(nt), float(h), float(dt)
+        self.device = torch.device(device)
+        self.K = int(ckpt_every)
+        self.nseg = 0 if self.K == 0 else (self.nt + self.K - 1) // self.K
+        self.models = [ModelGrid(nx, ny, order, self.device) for nx, ny in models]
+        self.shots = [dict(q) for q in shots]
+        S = self.S = len(self.shots)
+        if S == 0:
+            raise ValueError("a shot plan needs at least one shot")
+        dev = self.device
+        fs = [self.models[q["model"]].fstride for q in self.shots]
+        nrec = [int(q["nrec"]) for q in self.shots]
+        slots = self.nt if self.K == 0 else self.K
+        # arena offsets (floats); every field stride is a multiple of 32 floats
+        self.foff = [0] * S
+        self.roff = [0] * S
+        acc = racc = 0
+        for s in range(S):
+            self.foff[s], self.roff[s] = acc, racc
+            acc += fs[s]
+            racc += self.nt * nrec[s]
+        nf = acc
+        e = lambda n: torch.empty(n, dtype=torch.float32, device=dev)  # noqa: E731
+        self.u = torch.zeros(3 * nf, dtype=torch.float32, device=dev)
+        self.lam = torch.zeros(3 * nf, dtype=torch.float32, device=dev) if self.K else self.u
+        self.virt = e(slots * nf)
+        self.ckpt = e(2 * self.nseg * nf) if self.K else None
+        self.grad = torch.zeros(nf, dtype=torch.float32, device=dev)
+        self.frd = torch.zeros(racc, dtype=torch.float32, device=dev)
+        self.resid = torch.zeros(racc, dtype=torch.float32, device=dev)
+        self._wavelets = []
+        spec = _lib.TwPlanSpec(order=order, nt=self.nt, h=self.h, dt=self.dt, ckpt_every=self.K,
+                               tile_rows=int(tile_rows), graph_steps=int(graph_steps))
+        self.graph_steps = int(graph_steps)
+        cs = (_lib.TwShotSpec * S)()
+        wcache = {}
+        for s, q in enumerate(self.shots):
+            m = self.models[q["model"]]
+            f, o = fs[s], self.foff[s]
+            w = q["w"]
+            key = id(w)
+            if key not in wcache:
+                wd = torch.as_tensor(w)
+                if wd.dtype != torch.float32:
+                    raise RuntimeError("w: expected scalar type Float but found %s" % wd.dtype)
+                if wd.numel() < self.nt:
+                    raise RuntimeError("w has %d samples, needs nt=%d" % (wd.numel(), self.nt))
+                wcache[key] = wd.to(dev).contiguous()
+                self._wavelets.append(wcache[key])
+            c = cs[s]
+            c.dimx, c.dimy, c.ld = m.dimx, m.dimy, m.ld
+            c.sx, c.sy, c.ry = float(q["sx"]), float(q["sy"]), float(q["ry"])
+            c.rx0, c.nrec = int(q.get("rx0", 0)), nrec[s]
+            c.vel, c.vm, c.vc3 = m.vel.data_ptr(), m.vm.data_ptr(), m.vc3.data_ptr()
+            c.w = wcache[key].data_ptr()
+            # ring k of shot s lives at arena offset 3*o + k*f: contiguous 3 fields
+            c.u = self.u.data_ptr() + 4 * 3 * o
+            c.lam = self.lam.data_ptr() + 4 * 3 * o
+            c.frd = self.frd.data_ptr() + 4 * self.roff[s]
+            c.resid = self.resid.data_ptr() + 4 * self.roff[s]
+            c.virt = self.virt.data_ptr() + 4 * slots * o
+            c.ckpt = (self.ckpt.data_ptr() + 4 * 2 * self.nseg * o) if self.K else None
+            c.grad = self.grad.data_ptr() + 4 * o
+        self._fs, self._nrec = fs, nrec
+        L = _lib.lib()
+        nbytes = L.tw_plan_workspace_bytes(ctypes.byref(spec), cs, S)
+        if nbytes == 0:
+            raise RuntimeError("torchwi_amd: invalid shot plan: %s" % _lib.last_error())
+        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        handle = ctypes.c_void_p()
+        rc = L.tw_plan_create(ctypes.byref(spec), cs, S, _lib.ptr(self.workspace), nbytes,
+                              _lib.stream_of(self.workspace), ctypes.byref(handle))
+        _lib.check(rc, "tw_plan_create")
+        self.plan = handle
+
+    # ---- views -------------------------------------------------------------
+    def frd_of(self, s):
+        return self.frd[self.roff[s]:self.roff[s] + self.nt * self._nrec[s]].view(self.nt, self._nrec[s])
+
+    def resid_of(self, s):
+        return self.resid[self.roff[s]:self.roff[s] + self.nt * self._nrec[s]].view(self.nt, self._nrec[s])
+
+    def grad_of(self, s):
+        """Padded-grid gradient of shot s, (dimy, ld)."""
+        m = self.models[self.shots[s]["model"]]
+        return self.grad[self.foff[s]:self.foff[s] + self._fs[s]].view(m.dimy, m.ld)
+
+    def uniform(self):
+        return len(set(self._nrec)) == 1
+
+    def frd_all(self):
+        """(S, nt, nrec) view when every shot has the same receiver count."""
+        return self.frd.view(self.S, self.nt, self._nrec[0])
+
+    def resid_all(self):
+        return self.resid.view(self.S, self.nt, self._nrec[0])
+
+    # ---- execution ---------------------------------------------------------
+    def __del__(self):
+        try:
+            if getattr(self, "plan", None):
+                _lib.lib().tw_plan_destroy(self.plan)
+                self.plan = None
+        except Exception:
+            pass
+
+    def forward(self):
+        _lib.check(_lib.lib().tw_plan_forward(self.plan, _lib.stream_of(self.u)), "tw_plan_forward")
+
+    def backward(self, accumulate=False):
+        _lib.check(_lib.lib().tw_plan_backward(self.plan, int(bool(accumulate)),
+                                               _lib.stream_of(self.u)), "tw_plan_backward")
+
+    def set_timing(self, enable):
+        _lib.check(_lib.lib().tw_plan_set_timing(self.plan, int(bool(enable))), "tw_plan_set_timing")
+
+    def timing(self, reset=False):
+        """{kind: (total ms, launches)} of HIP-event-timed step kernels."""
+        n = len(_lib.KINDS)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_longlong * n)()
+        _lib.check(_lib.lib().tw_plan_timing(self.plan, int(bool(reset)), ms, cnt), "tw_plan_timing")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(_lib.KINDS)}
