@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--nt", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-launch HIP-event timing")
+    ap.add_argument("--time-block", type=int, default=1,
+                    help="time steps per launch (2..4: temporally blocked kernel)")
     ap.add_argument("--graph-steps", type=int, default=100,
                     help="time steps per captured hipGraph block (0: one direct launch per step)")
     ap.add_argument("--compare", type=int, default=1,
@@ -107,14 +109,14 @@ def cpu_baseline():
                       "forward with full storage + adjoint gradient; median of 5 runs (%.3f s each)" % t}
 
 
-def run_batched(c, K, steps, warmup, timing, graph_steps, dev):
+def run_batched(c, K, steps, warmup, timing, graph_steps, dev, time_block=1):
     """Time `steps` misfit gradients of the batched engine; returns stats."""
     import torch
     from torchwi_amd import Time2dBatch
     order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
     S = len(c["sx"])
     model = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K,
-                        graph_steps=graph_steps)
+                        graph_steps=graph_steps, time_block=time_block)
     vel = torch.from_numpy(c["vel"]).to(dev)
     rng = np.random.default_rng(7)
     vtrue = torch.from_numpy(c["vel"] * (1.0 + 0.02 * rng.standard_normal(c["vel"].shape)).astype(np.float32)).to(dev)
@@ -204,9 +206,10 @@ def main():
     probe = copy_probe(args.copy_probe) if args.copy_probe else None
     c = W.config2(nshots=args.nshots, nt=args.nt)
     order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
-    kck = args.ckpt_every or default_ckpt_interval(nt)
+    tb = args.time_block
+    kck = args.ckpt_every or default_ckpt_interval(nt, tb)
     K = {"full": 0, "ckpt": kck, "auto": "auto"}[args.mode]
-    r = run_batched(c, K, args.steps, args.warmup, not args.no_timing, args.graph_steps, dev)
+    r = run_batched(c, K, args.steps, args.warmup, not args.no_timing, args.graph_steps, dev, tb)
     S, kstate, value, elapsed, lat = r["S"], r["K"], r["value"], r["elapsed"], r["lat"]
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
@@ -219,7 +222,7 @@ def main():
                                % (S, ny, nx, r["dimy"], r["dimx"], h, dt, nt, order,
                                   "full virtual-source storage" if kstate == 0 else "checkpoint every %d steps + recompute" % kstate),
                    "shots": S, "nz": ny, "nx": nx, "nt": nt, "order": order, "ckpt_every": kstate,
-                   "graph_steps": args.graph_steps},
+                   "graph_steps": args.graph_steps, "time_block": tb},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "gradient_shots_per_s": round(r["shots_per_s"], 3),
         "effective_hbm_frac": round(value * 1e9 * (BYTES_GRAD_STEP / 2.0) / (HBM_PEAK_GBS * 1e9), 4),
@@ -238,13 +241,14 @@ def main():
         dom = max(per, key=lambda k: per[k]["avg_us"] * per[k]["launches"])
         b = {"fwd": BYTES_FWD, "fwd_virt": BYTES_FWD_VIRT, "recomp": BYTES_FWD_VIRT, "adj": BYTES_ADJ}[dom]
         ach = per[dom]["achieved_GBs"]
-        tr = load_traffic(dom) if (nt, S, order) == (4000, 32, 8) else None
-        out["roofline"] = {"bound": "hbm", "kernel": "k_vstep<4,%s>" % dom, "achieved": ach,
+        tr = load_traffic(dom if tb <= 1 else "%s_tb%d" % (dom, tb)) if (nt, S, order) == (4000, 32, 8) else None
+        out["roofline"] = {"bound": "hbm", "kernel": ("k_tstep<4,%s,%d>" % (dom, tb)) if tb > 1 else "k_vstep<4,%s>" % dom,
+                           "achieved": ach,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                            "traffic": tr, "bytes_per_cell": b, "cells_per_launch": cells_launch}
     if args.compare:
         if kstate == 0:
-            rc = run_batched(c, kck, 1, 0, False, args.graph_steps, dev)
+            rc = run_batched(c, kck, 1, 0, False, args.graph_steps, dev, tb)
             out["checkpointed"] = {"ckpt_every": rc["K"], "value": round(rc["value"], 3),
                                    "ms_per_step": round(rc["elapsed"] * 1e3, 3),
                                    "gradient_shots_per_s": round(rc["shots_per_s"], 3),

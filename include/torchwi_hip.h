@@ -113,6 +113,12 @@ typedef struct TwPlanSpec {
                          captured once per plan (this many steps per forward /
                          full-storage adjoint block; one graph per checkpoint
                          segment); 0: one direct launch per step */
+    int time_block;   /* 2..4: temporally blocked kernel advancing this many
+                         steps per launch (ckpt_every must be a multiple);
+                         0 or 1: one step per launch */
+    int forward_only; /* 1: modelling only -- no virtual-source storage, no
+                         adjoint ring (virt, lam, ckpt may be NULL; ckpt_every
+                         must be 0; tw_plan_backward is refused) */
 } TwPlanSpec;
 
 typedef struct TwShotSpec {
@@ -125,8 +131,8 @@ typedef struct TwShotSpec {
     float *vm, *vc3;  /* (dimy, ld) derived-model work arrays, filled by the
                          plan; shots sharing vel share them */
     const float *w;   /* (nt) wavelet */
-    float *u;         /* forward ring: 3 fields of dimy*ld */
-    float *lam;       /* adjoint ring: 3 fields of dimy*ld (may equal u when
+    float *u;         /* forward ring: 4 fields of dimy*ld */
+    float *lam;       /* adjoint ring: 4 fields of dimy*ld (may equal u when
                          ckpt_every == 0) */
     float *frd;       /* (nt, nrec) gather */
     const float *resid; /* (nt, nrec) residual, read by tw_plan_backward */

@@ -319,3 +319,34 @@ def test_windows_match_oracle_on_cropped_models():
         assert np.array_equal(got, g_o), f"window {s} gradient"
         want[q["x0"]:q["x1"], :] += g_o
     assert rel_l2(g.cpu().numpy(), want) < 1e-6
+
+
+@pytest.mark.parametrize("order,T,nx,ny,nt,K", [
+    (8, 3, 150, 70, 100, 0), (8, 3, 150, 70, 101, 12), (8, 2, 61, 130, 77, 8), (8, 4, 200, 64, 90, 16),
+    (4, 3, 131, 57, 95, 9), (2, 3, 67, 29, 80, 0), (2, 4, 70, 150, 60, 8), (8, 3, 20, 12, 40, 6)])
+def test_temporal_blocking_bitexact(order, T, nx, ny, nt, K):
+    """The temporally blocked kernel (T steps per launch, redundant halo)
+    gives the same gathers and per-shot gradients as one step per launch,
+    which equal the oracle; with checkpoints (K a multiple of T) too."""
+    from torchwi_amd import Time2dBatch
+    c = small_case(order, nx=nx, ny=ny, nt=nt, seed=nx * 7 + T)
+    sx = [(i + 0.5) * c["h"] for i in (1, nx // 2, nx - 2)]
+    sy = [(i + 0.5) * c["h"] for i in (0, ny // 3, ny - 1)]
+    ry = [(i + 0.5) * c["h"] for i in (1, 0, ny // 2)]
+    print(f"case order={order} T={T} nx={nx} ny={ny} nt={nt} K={K}", flush=True)
+    vel = torch.from_numpy(c["vel"]).to(DEV)
+    w = torch.from_numpy(c["w"])
+    res = []
+    for tb in (1, T):
+        m = Time2dBatch(nx, ny, c["h"], w, c["dt"], order, DEV, ckpt_every=K, time_block=tb)
+        frd = m.modeling(vel, sx, sy, ry).clone()
+        loss, g = m.misfit_gradient(vel, sx, sy, ry, 0.6 * frd)
+        res.append((frd, g.clone(), [m.plan.grad_of(s).clone() for s in range(3)], float(loss)))
+    assert torch.equal(res[0][0], res[1][0]), "gathers"
+    for s in range(3):
+        assert torch.equal(res[0][2][s], res[1][2][s]), f"shot {s} gradient"
+    assert torch.equal(res[0][1], res[1][1])
+    vpad = O.pad_vel(c["vel"], order)
+    frd_o, _ = O.forward(vpad, c["w"], order, c["dimx"], c["dimy"], c["h"], c["dt"], sx[1], sy[1], ry[1],
+                         store_virt=False)
+    assert np.array_equal(res[1][0][1].cpu().numpy(), frd_o)

@@ -26,6 +26,14 @@ def test_gradient_chunks_with_checkpoints_under_a_small_cap():
     assert (per, k2, b) == (3, K, three)
 
 
+def test_forward_requests_hold_no_virtual_sources():
+    r = _req("forward", nt=900, S=4)
+    per, K, b = chunk_plan(r, 10 ** 12)
+    assert (per, K) == (4, 0)
+    assert b == plan_bytes([(r.nx, r.ny, 8)], [dict(model=0, nrec=r.nx)] * 4, r.nt, 0, forward_only=True)
+    assert b < plan_bytes([(r.nx, r.ny, 8)], [dict(model=0, nrec=r.nx)] * 4, r.nt, 0) / 50
+
+
 def test_chunking_depends_on_the_request_only():
     r = _req("forward", S=7)
     assert chunk_plan(r, 10 ** 9) == chunk_plan(_req("forward", S=7), 10 ** 9)

@@ -19,7 +19,7 @@ gradients in shot order.
 """
 import torch
 
-from .shots import GRAPH_STEPS, ShotPlan, default_ckpt_interval, plan_bytes
+from .shots import GRAPH_STEPS, TIME_BLOCK, ShotPlan, default_ckpt_interval, plan_bytes
 
 __all__ = ["Time2dBatch", "Time2dWindows", "default_ckpt_interval"]
 
@@ -39,13 +39,13 @@ def _wavelet(w):
     return w.contiguous()
 
 
-def _choose_ckpt(ckpt_every, need_full, nt, device, hbm_fraction, held=0):
+def _choose_ckpt(ckpt_every, need_full, nt, device, hbm_fraction, held=0, time_block=1):
     if ckpt_every != "auto":
         return int(ckpt_every)
     free, _ = torch.cuda.mem_get_info(device)
     if need_full <= hbm_fraction * (free + held):
         return 0
-    return default_ckpt_interval(nt)
+    return default_ckpt_interval(nt, time_block)
 
 
 class _BatchOperator(torch.autograd.Function):
@@ -80,12 +80,13 @@ class Time2dBatch(torch.nn.Module):
     """
 
     def __init__(self, nx, ny, h, w, dt, order, device="cuda", ckpt_every="auto",
-                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS):
+                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK):
         super().__init__()
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise NotImplementedError("Time2dBatch runs on the GPU only")
         self.graph_steps = graph_steps
+        self.time_block = time_block
         if order not in (2, 4, 8):
             raise ValueError("Wrong order: use 2, 4, and 8")
         self.nx, self.ny, self.order, self.ne = nx, ny, order, order // 2
@@ -111,11 +112,12 @@ class Time2dBatch(torch.nn.Module):
             models = [(self.nx, self.ny, self.order)]
             held = plan_bytes(models, self._plan.shots, self.nt, self._plan.K) if self._plan else 0
             K = _choose_ckpt(self.ckpt_every, plan_bytes(models, shots, self.nt, 0), self.nt,
-                             self.device, self.hbm_fraction, held)
+                             self.device, self.hbm_fraction, held, self.time_block)
             self._plan = None
             torch.cuda.empty_cache()
             self._plan = ShotPlan(self.order, self.nt, self.h, self.dt, [(self.nx, self.ny)], shots, K,
-                                  self.device, tile_rows=self.tile_rows, graph_steps=self.graph_steps)
+                                  self.device, tile_rows=self.tile_rows, graph_steps=self.graph_steps,
+                                  time_block=self.time_block)
             self._key = key
         self._plan.models[0].load(vel)
         return self._plan
@@ -170,7 +172,7 @@ class Time2dWindows(torch.nn.Module):
     """
 
     def __init__(self, nx, ny, h, w, dt, order, windows, device="cuda", ckpt_every="auto",
-                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS):
+                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK):
         super().__init__()
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -187,9 +189,9 @@ class Time2dWindows(torch.nn.Module):
         shots = [dict(model=i, sx=q["sx"], sy=q["sy"], ry=q["ry"], rx0=q["rx0"], nrec=q["nrec"],
                       w=q.get("w", self.w)) for i, q in enumerate(self.windows)]
         K = _choose_ckpt(ckpt_every, plan_bytes([(a, b, order) for a, b in models], shots, self.nt, 0),
-                         self.nt, self.device, hbm_fraction)
+                         self.nt, self.device, hbm_fraction, time_block=time_block)
         self.plan = ShotPlan(order, self.nt, h, dt, models, shots, K, self.device, tile_rows=tile_rows,
-                             graph_steps=graph_steps)
+                             graph_steps=graph_steps, time_block=time_block)
 
     def load(self, vel):
         for i, q in enumerate(self.windows):

@@ -73,6 +73,55 @@ __device__ __forceinline__ float interior_update(const float* T, int ts, float u
     }
 }
 
+// The same update for four consecutive cells x..x+3 of one row, reading the
+// staged field as float4s: rows y-k and y+k at x..x+3 (2*NE reads) and the
+// centre row at x-4..x+7 (3 reads).  T must be 16-byte aligned and the row
+// stride ts a multiple of 4.  The per-cell arithmetic and its order are
+// exactly interior_update's.
+__device__ __forceinline__ float f4at(const float4& v, int j)
+{
+    return j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
+}
+
+template <int NE>
+__device__ __forceinline__ float4 interior_update4(const float* T, int ts, float4 u1, float4 m)
+{
+    const float4 L = *(const float4*)(T - 4);
+    const float4 C = *(const float4*)(T);
+    const float4 Rr = *(const float4*)(T + 4);
+    float4 up[NE + 1], dn[NE + 1];
+#pragma unroll
+    for (int k = 1; k <= NE; ++k) {
+        up[k] = *(const float4*)(T - k * ts);
+        dn[k] = *(const float4*)(T + k * ts);
+    }
+    // centre-row value at offset o in [-4, 7]
+    auto row = [&](int o) -> float { return o < 0 ? f4at(L, o + 4) : (o < 4 ? f4at(C, o) : f4at(Rr, o - 4)); };
+    float out[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float c = f4at(C, j);
+        const float um = f4at(u1, j), mm = f4at(m, j);
+        if constexpr (NE == 1) {
+            float s = ((row(j - 1) + f4at(up[1], j)) + row(j + 1)) + f4at(dn[1], j);
+            float lap = (float)(LapW<1>::w(1) * (double)s + (LapW<1>::w(0) * 2.0) * (double)c);
+            out[j] = (2.f * c - um) + mm * lap;
+        } else {
+            float s = ((f4at(up[NE], j) + row(j - NE)) + row(j + NE)) + f4at(dn[NE], j);
+            double acc = LapW<NE>::w(NE) * (double)s;
+#pragma unroll
+            for (int k = NE - 1; k >= 1; --k) {
+                s = ((f4at(up[k], j) + row(j - k)) + row(j + k)) + f4at(dn[k], j);
+                acc = acc + LapW<NE>::w(k) * (double)s;
+            }
+            acc = acc + (LapW<NE>::w(0) * 2.0) * (double)c;
+            float lap = (float)acc;
+            out[j] = (lap * mm + 2.f * c) - um;
+        }
+    }
+    return make_float4(out[0], out[1], out[2], out[3]);
+}
+
 // Keys update of a boundary cell from its own (um0, uo0) and its first and
 // second inward neighbours (um1, uo1, uo2).
 __device__ __forceinline__ float keys_update(float um0, float uo0, float um1, float uo1,

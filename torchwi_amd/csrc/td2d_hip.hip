@@ -57,8 +57,8 @@ struct ShotDesc {
     const float* vm;          // (v*v)*dtoh2 per cell (batched path)
     const float* vc3;         // cons/((v*v)*v) per cell (batched path)
     const float* w;
-    float* u[3];              // forward ring
-    float* lam[3];            // adjoint ring
+    float* u[4];              // forward ring (3 fields per step launch, two pairs per block launch)
+    float* lam[4];            // adjoint ring
     float* frd;
     const float* resid;
     float* virt;
@@ -148,7 +148,7 @@ __device__ __forceinline__ int logical_tile(int b, int per_xcd)
 
 // Select one of three ring pointers without dynamic indexing (which would
 // force a by-value descriptor into scratch memory).
-__device__ __forceinline__ float* pick3(float* const (&r)[3], int k)
+__device__ __forceinline__ float* pick3(float* const (&r)[4], int k)
 {
     return k == 0 ? r[0] : (k == 1 ? r[1] : r[2]);
 }
@@ -253,6 +253,7 @@ __global__ __launch_bounds__(NTHR) void k_step_single(ShotDesc d, StepArgs a)
 }  // namespace tw
 
 #include "td2d_vec.hpp"
+#include "td2d_temporal.hpp"
 
 namespace tw {
 
@@ -276,7 +277,7 @@ __global__ void k_zero_i16(short* __restrict__ p, long long n)
 // step t0: u_prev <- ckpt[seg][0], u_cur <- ckpt[seg][1] (zero for seg 0);
 // or (what = 1) zero the whole adjoint ring; or (what = 2) zero grad.
 __global__ void k_batch_prepare(const ShotDesc* __restrict__ descs, int what, int seg, int t0,
-                                const int* __restrict__ ctl, int K)
+                                const int* __restrict__ ctl, int K, int pairs)
 {
     if (ctl) {  // segment index from the control word (graph replay)
         seg = *ctl;
@@ -287,8 +288,10 @@ __global__ void k_batch_prepare(const ShotDesc* __restrict__ descs, int what, in
     const long long stride = (long long)gridDim.x * blockDim.x;
     const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (what == 0) {
-        float* p0 = pick3(d.u, t0 % 3);
-        float* p1 = pick3(d.u, (t0 + 1) % 3);
+        // one-step launches rotate three fields by step; block launches read
+        // the pair (u[0], u[1]) first
+        float* p0 = pairs ? d.u[0] : pick3(d.u, t0 % 3);
+        float* p1 = pairs ? d.u[1] : pick3(d.u, (t0 + 1) % 3);
         if (seg == 0) {
             for (long long i = i0; i < n; i += stride) { p0[i] = 0.f; p1[i] = 0.f; }
         } else {
@@ -299,6 +302,7 @@ __global__ void k_batch_prepare(const ShotDesc* __restrict__ descs, int what, in
     } else if (what == 1) {
         for (long long i = i0; i < n; i += stride) {
             d.lam[0][i] = 0.f; d.lam[1][i] = 0.f; d.lam[2][i] = 0.f;
+            if (pairs) d.lam[3][i] = 0.f;
         }
     } else {
         for (long long i = i0; i < n; i += stride) d.grad[i] = 0.f;
@@ -612,6 +616,8 @@ struct Graph {
 
 struct Plan {
     Timer timer;
+    bool fwd_only = false;            // modelling only: no virtual sources, no adjoint
+    int tb = 1;                       // time steps per launch (> 1: temporally blocked kernel)
     int graph_steps = 0;              // steps per captured block (0: direct launches)
     hipStream_t cap = nullptr;        // private stream graphs are captured on
     Graph g_fwd, g_bwd, g_recomp, g_adj;
@@ -662,6 +668,13 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
     if (p.K < 0) return fail(TW_EINVAL, "ckpt_every must be >= 0");
     if (spec->graph_steps < 0 || spec->graph_steps > 4096)
         return fail(TW_EINVAL, "graph_steps must be in [0, 4096] (got %d)", spec->graph_steps);
+    p.fwd_only = spec->forward_only != 0;
+    if (p.fwd_only && p.K != 0) return fail(TW_EINVAL, "forward_only plans take ckpt_every = 0");
+    p.tb = spec->time_block <= 1 ? 1 : spec->time_block;
+    if (p.tb != 1 && p.tb != 2 && p.tb != 3 && p.tb != 4)
+        return fail(TW_EINVAL, "time_block must be 0..4 (got %d)", spec->time_block);
+    if (p.tb > 1 && p.K > 0 && p.K % p.tb != 0)
+        return fail(TW_EINVAL, "ckpt_every=%d must be a multiple of time_block=%d", p.K, p.tb);
     p.nseg = p.K > 0 ? (p.nt + p.K - 1) / p.K : 0;
     p.nshots = nshots;
     const int ne = spec->order / 2;
@@ -685,13 +698,15 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
         if (q.rx0 < 0 || q.nrec <= 0 || q.rx0 + q.nrec > nx)
             return fail(TW_EINVAL, "shot %d: receiver window [%d,%d) outside [0,%d)", s, q.rx0,
                         q.rx0 + q.nrec, nx);
-        if (!q.vel || !q.vm || !q.vc3 || !q.w || !q.u || !q.lam || !q.frd || !q.grad || !q.virt)
+        if (!q.vel || !q.vm || !q.vc3 || !q.w || !q.u || !q.frd)
             return fail(TW_EINVAL, "shot %d: NULL buffer", s);
+        if (!p.fwd_only && (!q.lam || !q.grad || !q.virt))
+            return fail(TW_EINVAL, "shot %d: NULL adjoint buffer", s);
         if (p.K > 0 && !q.ckpt) return fail(TW_EINVAL, "shot %d: NULL checkpoint buffer", s);
         if (p.K > 0 && q.lam == q.u)
             return fail(TW_EINVAL, "shot %d: checkpointing needs separate forward and adjoint rings", s);
         if (!al16(q.vel) || !al16(q.vm) || !al16(q.vc3) || !al16(q.u) || !al16(q.lam) ||
-            !al16(q.virt) || !al16(q.grad) || (q.ckpt && !al16(q.ckpt)))
+            !al16(q.virt) || !al16(q.grad) || !al16(q.ckpt))
             return fail(TW_EINVAL, "shot %d: field buffers must be 16-byte aligned", s);
         ShotDesc& d = p.host[s];
         std::memset(&d, 0, sizeof d);
@@ -709,9 +724,9 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
         d.vm = q.vm;
         d.vc3 = q.vc3;
         d.w = q.w;
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 4; ++k) {
             d.u[k] = q.u + k * d.fstride;
-            d.lam[k] = q.lam + k * d.fstride;
+            d.lam[k] = q.lam ? q.lam + k * d.fstride : nullptr;
         }
         d.frd = q.frd;
         d.resid = q.resid;
@@ -734,7 +749,7 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
     if (spec->tile_rows == 32 || spec->tile_rows == 64) p.ryv = spec->tile_rows / 16;
     else if (spec->tile_rows == 0) p.ryv = choose_ryv(dimys, ne);
     else return fail(TW_EINVAL, "tile_rows must be 0, 32 or 64 (got %d)", spec->tile_rows);
-    const int ty = 16 * p.ryv;
+    const int ty = p.tb > 1 ? TCY : 16 * p.ryv;
     // Deal tiles model by model; within a model spatial-tile-major and
     // shot-minor, so consecutive workgroups of an XCD share model rows.
     for (const ModelRef& m : p.models) {
@@ -839,7 +854,7 @@ int capture(Plan& p, Graph& gr, int which)
     a.B = p.graph_steps;
     if (which == 0) {         // forward block: B steps, then ctl0 += 1
         a.phase = PH_FWD_BLK;
-        a.flags = p.K == 0 ? M_FWD_VIRT : M_FWD;
+        a.flags = (p.K == 0 && !p.fwd_only) ? M_FWD_VIRT : M_FWD;
         for (int j = 0; j < p.graph_steps && !rc; ++j) { a.j = j; rc = launch_batch(p, a, p.cap, false); }
         if (!rc) rc = set_ctl(p, 0, 1, 1, p.cap);
     } else if (which == 1) {  // full-storage adjoint block: B steps, then ctl0 += 1
@@ -892,6 +907,68 @@ int replay(Plan& p, Graph& gr, int kind, int steps, hipStream_t s)
     return TW_OK;
 }
 
+template <int NE, unsigned FLAGS, int T>
+void launch_t_nft(const Plan& p, const TArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_tstep<NE, FLAGS, T>), dim3(p.per_xcd * NXCD), dim3(TTHR), 0, s, p.d_desc,
+                       p.d_tiles, a);
+}
+
+template <int NE, unsigned FLAGS>
+void launch_t_nf(const Plan& p, const TArgs& a, hipStream_t s)
+{
+    if (p.tb == 2) launch_t_nft<NE, FLAGS, 2>(p, a, s);
+    else if (p.tb == 3) launch_t_nft<NE, FLAGS, 3>(p, a, s);
+    else launch_t_nft<NE, FLAGS, 4>(p, a, s);
+}
+
+template <int NE>
+int launch_t_ne(const Plan& p, const TArgs& a, unsigned flags, hipStream_t s)
+{
+    switch (flags) {
+    case M_FWD: launch_t_nf<NE, M_FWD>(p, a, s); break;
+    case M_FWD_VIRT: launch_t_nf<NE, M_FWD_VIRT>(p, a, s); break;
+    case M_RECOMP: launch_t_nf<NE, M_RECOMP>(p, a, s); break;
+    case M_ADJ: launch_t_nf<NE, M_ADJ>(p, a, s); break;
+    default: return fail(TW_EINVAL, "internal: no blocked step variant for flags %u", flags);
+    }
+    return TW_OK;
+}
+
+// Launch one temporally blocked step of `n` levels starting at time t
+// (advancing by dir), reading field pair in_pair and writing the other.
+int launch_block(Plan& p, unsigned flags, int t, int dir, int n, int in_pair, int vslot, int adj,
+                 hipStream_t s)
+{
+    TArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.t_first = t;
+    a.dir = dir;
+    a.nsteps = n;
+    a.in_pair = in_pair;
+    a.out_pair = in_pair ^ 1;
+    a.vslot_first = vslot;
+    a.adj = adj;
+    a.K = p.K;
+    a.nseg = p.nseg;
+    a.ntiles = p.ntiles;
+    a.per_xcd = p.per_xcd;
+    EventPair* ev = p.timer.enabled ? p.timer.next(s) : nullptr;
+    if (ev) TW_HIP(hipEventRecord(ev->start, s));
+    int rc;
+    if (p.order == 2) rc = launch_t_ne<1>(p, a, flags, s);
+    else if (p.order == 4) rc = launch_t_ne<2>(p, a, flags, s);
+    else rc = launch_t_ne<4>(p, a, flags, s);
+    if (rc) return rc;
+    TW_HIP(hipGetLastError());
+    if (ev) {
+        TW_HIP(hipEventRecord(ev->stop, s));
+        ev->kind = timing_kind(flags);
+        ev->steps = n;
+    }
+    return TW_OK;
+}
+
 // Recompute the derived arrays of every model of the plan.
 int prep_models(const Plan& p, hipStream_t s)
 {
@@ -911,7 +988,7 @@ int batch_prepare(Plan& p, int what, int seg, int t0, hipStream_t s)
     long long bx = (p.max_field + 255) / 256;
     if (bx > 1024) bx = 1024;
     hipLaunchKernelGGL(k_batch_prepare, dim3((unsigned)bx, p.nshots), dim3(256), 0, s,
-                       p.d_desc, what, seg, t0, (const int*)nullptr, p.K);
+                       p.d_desc, what, seg, t0, (const int*)nullptr, p.K, p.tb > 1 ? 1 : 0);
     TW_HIP(hipGetLastError());
     return TW_OK;
 }
@@ -922,7 +999,7 @@ int batch_prepare_ctl(Plan& p, hipStream_t s)
     long long bx = (p.max_field + 255) / 256;
     if (bx > 1024) bx = 1024;
     hipLaunchKernelGGL(k_batch_prepare, dim3((unsigned)bx, p.nshots), dim3(256), 0, s,
-                       p.d_desc, 0, 0, 0, (const int*)(p.d_ctl + 1), p.K);
+                       p.d_desc, 0, 0, 0, (const int*)(p.d_ctl + 1), p.K, 0);
     TW_HIP(hipGetLastError());
     return TW_OK;
 }
@@ -1069,7 +1146,17 @@ int tw_plan_forward(void* plan, void* stream)
     // zero the forward ring: prepare(seg 0) zeroes two fields, the third is
     // written completely by the first step.
     if ((rc = batch_prepare(p, 0, 0, 0, s))) return rc;
-    const int kind = p.K == 0 ? TW_KIND_FWD_VIRT : TW_KIND_FWD;
+    const bool store = p.K == 0 && !p.fwd_only;
+    const int kind = store ? TW_KIND_FWD_VIRT : TW_KIND_FWD;
+    if (p.tb > 1) {
+        const unsigned flags = store ? M_FWD_VIRT : M_FWD;
+        int b = 0;
+        for (int t0 = 0; t0 < p.nt; t0 += p.tb, ++b) {
+            const int n = p.nt - t0 < p.tb ? p.nt - t0 : p.tb;
+            if ((rc = launch_block(p, flags, t0, +1, n, b & 1, t0, 0, s))) return rc;
+        }
+        return TW_OK;
+    }
     if (p.graph_steps > 0) {
         if ((rc = capture(p, p.g_fwd, 0))) return rc;
         if ((rc = set_ctl(p, 0, 0, 0, s))) return rc;
@@ -1086,7 +1173,7 @@ int tw_plan_forward(void* plan, void* stream)
         a.adj = 0;
         a.flags = M_FWD;
         a.ck0 = a.ck1 = -1;
-        if (p.K == 0) {
+        if (store) {
             a.flags = M_FWD_VIRT;
             a.vslot = it;
         } else {
@@ -1104,11 +1191,38 @@ int tw_plan_backward(void* plan, int accumulate, void* stream)
     Plan& p = *(Plan*)plan;
     hipStream_t s = (hipStream_t)stream;
     int rc;
+    if (p.fwd_only) return fail(TW_EINVAL, "this plan was built forward_only");
     for (int k = 0; k < p.nshots; ++k)
         if (!p.host[k].resid) return fail(TW_EINVAL, "shot %d: NULL residual", k);
     if ((rc = prep_models(p, s))) return rc;
     if (!accumulate && (rc = batch_prepare(p, 2, 0, 0, s))) return rc;
     if ((rc = batch_prepare(p, 1, 0, 0, s))) return rc;
+    if (p.tb > 1) {
+        int ab = 0;  // adjoint blocks done: parity of the adjoint field pair
+        if (p.K == 0) {
+            for (int s0 = 0; s0 < p.nt; s0 += p.tb, ++ab) {
+                const int n = p.nt - s0 < p.tb ? p.nt - s0 : p.tb;
+                const int it = p.nt - 1 - s0;
+                if ((rc = launch_block(p, M_ADJ, it, -1, n, ab & 1, it, 1, s))) return rc;
+            }
+            return TW_OK;
+        }
+        for (int seg = p.nseg - 1; seg >= 0; --seg) {
+            const int t0 = seg * p.K;
+            const int t1 = (t0 + p.K < p.nt) ? t0 + p.K : p.nt;
+            if ((rc = batch_prepare(p, 0, seg, t0, s))) return rc;
+            int b = 0;
+            for (int t = t0; t < t1; t += p.tb, ++b) {
+                const int n = t1 - t < p.tb ? t1 - t : p.tb;
+                if ((rc = launch_block(p, M_RECOMP, t, +1, n, b & 1, t - t0, 0, s))) return rc;
+            }
+            for (int it = t1 - 1; it >= t0; it -= p.tb, ++ab) {
+                const int n = it - t0 + 1 < p.tb ? it - t0 + 1 : p.tb;
+                if ((rc = launch_block(p, M_ADJ, it, -1, n, ab & 1, it - t0, 1, s))) return rc;
+            }
+        }
+        return TW_OK;
+    }
     if (p.graph_steps > 0) {
         if (p.K == 0) {
             if ((rc = capture(p, p.g_bwd, 1))) return rc;

@@ -36,6 +36,19 @@ __device__ __forceinline__ void gst4(float* p, float4 v)
     f4v t = {v.x, v.y, v.z, v.w};
     *(gf4v*)p = t;
 }
+// Streaming (nontemporal) variants for data touched once per pass: virtual
+// sources (written by the forward pass, read once by the adjoint) and
+// checkpoints.
+__device__ __forceinline__ float4 gld4_nt(const float* p)
+{
+    const f4v v = __builtin_nontemporal_load((const gf4v*)p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst4_nt(float* p, float4 v)
+{
+    f4v t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, (gf4v*)p);
+}
 __device__ __forceinline__ float4 sel4(bool c, float4 a)
 {
     return make_float4(c ? a.x : 0.f, c ? a.y : 0.f, c ? a.z : 0.f, c ? a.w : 0.f);
@@ -97,7 +110,7 @@ __global__ __launch_bounds__(VTHR) void k_vstep(const ShotDesc* __restrict__ des
         pm[r] = gld4(d.vm + i);
         if constexpr ((FLAGS & F_VIRT) != 0) pc3[r] = gld4(d.vc3 + i);
         if constexpr ((FLAGS & F_GRAD) != 0) {
-            pvs[r] = gld4(vslot + i);
+            pvs[r] = gld4_nt(vslot + i);
             pg[r] = gld4(d.grad + i);
         }
     }
@@ -123,10 +136,7 @@ __global__ __launch_bounds__(VTHR) void k_vstep(const ShotDesc* __restrict__ des
         const float* Tc = Tf + (ty * RYV + r + HALO) * W + 4 * tx + 4;
         float4 un = z4;
         if (xin && gy >= NE && gy < dimy - NE) {
-            un.x = interior_update<NE>(Tc + 0, W, pum[r].x, pm[r].x);
-            un.y = interior_update<NE>(Tc + 1, W, pum[r].y, pm[r].y);
-            un.z = interior_update<NE>(Tc + 2, W, pum[r].z, pm[r].z);
-            un.w = interior_update<NE>(Tc + 3, W, pum[r].w, pm[r].w);
+            un = interior_update4<NE>(Tc, W, pum[r], pm[r]);
         } else {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -193,7 +203,7 @@ __global__ __launch_bounds__(VTHR) void k_vstep(const ShotDesc* __restrict__ des
             vs.y = pc3[r].y * ((pum[r].y - 2.f * Tc[1]) + un.y);
             vs.z = pc3[r].z * ((pum[r].z - 2.f * Tc[2]) + un.z);
             vs.w = pc3[r].w * ((pum[r].w - 2.f * Tc[3]) + un.w);
-            gst4(d.virt + a.vslot * d.fstride + i, vs);
+            gst4_nt(d.virt + a.vslot * d.fstride + i, vs);
         }
         if constexpr ((FLAGS & F_GRAD) != 0) {
             float4 g;
@@ -203,8 +213,8 @@ __global__ __launch_bounds__(VTHR) void k_vstep(const ShotDesc* __restrict__ des
             g.w = pg[r].w + (-pvs[r].w) * un.w;
             gst4(d.grad + i, g);
         }
-        if (a.ck0 >= 0) gst4(d.ckpt + (2LL * a.ck0) * d.fstride + i, un);
-        if (a.ck1 >= 0) gst4(d.ckpt + (2LL * a.ck1 + 1) * d.fstride + i, un);
+        if (a.ck0 >= 0) gst4_nt(d.ckpt + (2LL * a.ck0) * d.fstride + i, un);
+        if (a.ck1 >= 0) gst4_nt(d.ckpt + (2LL * a.ck1 + 1) * d.fstride + i, un);
     }
 }
 

@@ -22,6 +22,7 @@ Prints one JSON line.
 """
 import argparse
 import json
+import sys
 import time
 
 import numpy as np
@@ -42,6 +43,10 @@ def direction(nx, ny, seed, sigma=20.0, margin=10):
     d[:, nx - margin:] = 0
     d[ny - margin:, :] = 0
     return np.ascontiguousarray(d.T.astype(np.float32))
+
+
+def progress(msg):
+    print("diffcheck: " + msg, file=sys.stderr, flush=True)
 
 
 def misfit64(frd, d_obs):
@@ -90,6 +95,7 @@ def main(argv=None):
         torch.cuda.synchronize()
         out["full_storage_1shot_s"] = round(time.perf_counter() - t0, 3)
         g0_full = mf.plan.grad_of(0).clone()
+        progress("full-storage shot-0 gradient done (%.2f s)" % out["full_storage_1shot_s"])
         out["full_storage_bytes_per_shot"] = int(mf.plan.virt.numel() * 4)
         del mf
         torch.cuda.empty_cache()
@@ -99,6 +105,7 @@ def main(argv=None):
     d_obs = m.modeling(vtrue, c["sx"], c["sy"], c["ry"]).clone()
     torch.cuda.synchronize()
     out["forward_s"] = round(time.perf_counter() - t0, 3)
+    progress("observed data modelled (%.2f s)" % out["forward_s"])
     if d0 is not None:
         out["batched_gather_equals_single_shot0"] = bool(torch.equal(d_obs[0], d0[0]))
 
@@ -110,6 +117,7 @@ def main(argv=None):
     out.update(gradient_s=round(tg, 3), gradient_Gcell_updates_per_s=round(cells / tg / 1e9, 2),
                checkpoint_plan_GB=round(sum(t.numel() for t in (m.plan.u, m.plan.lam, m.plan.virt, m.plan.ckpt,
                                                                    m.plan.grad)) * 4 / 1e9, 2))
+    progress("checkpointed gradient of %d shots done (%.2f s)" % (args.shots, tg))
     if g0_full is not None:
         out["checkpointed_equals_full_storage_shot0"] = bool(torch.equal(m.plan.grad_of(0), g0_full))
     g = g.clone()
@@ -124,6 +132,7 @@ def main(argv=None):
         fd = (Js[0] - Js[1]) / (2 * args.eps)
         an = float(torch.sum((g * vel * d).double()))
         frd_dirs.append({"seed": 10 + k, "fd": fd, "adjoint": an, "rel_err": abs(fd - an) / abs(an)})
+        progress("direction %d: fd %.6e adjoint %.6e rel %.2e" % (k, fd, an, frd_dirs[-1]["rel_err"]))
     out["fd_checks"] = frd_dirs
     out["max_fd_rel_err"] = max(x["rel_err"] for x in frd_dirs) if frd_dirs else None
     print(json.dumps(out), flush=True)

@@ -18,6 +18,7 @@ There is no counterpart in the reference, whose shot loop is one process per
 GPU under Horovod (examples/time/inv_torch.py:123-143).
 """
 import math
+import sys
 import time
 from dataclasses import dataclass, field
 from typing import List, Optional
@@ -89,8 +90,10 @@ def chunk_plan(req, cap_bytes):
     models = [(req.nx, req.ny, req.order)]
     shot = dict(model=0, nrec=req.nx)
 
+    fwd = req.kind == "forward"
+
     def nbytes(n, K):
-        return plan_bytes(models, [shot] * n, req.nt, K)
+        return plan_bytes(models, [shot] * n, req.nt, K, forward_only=fwd)
 
     if req.kind == "gradient" and nbytes(req.nshots, 0) <= cap_bytes:
         # full storage when the whole request fits under the cap: less traffic
@@ -129,7 +132,8 @@ class Server:
     cap_bytes: per-request chunk cap (default budget/slots)."""
 
     def __init__(self, device="cuda", budget_bytes=None, slots=4, cap_bytes=None,
-                 graph_steps=GRAPH_STEPS):
+                 graph_steps=GRAPH_STEPS, verbose=False):
+        self.verbose = verbose
         self.device = torch.device(device)
         if budget_bytes is None:
             free, _ = torch.cuda.mem_get_info(self.device)
@@ -202,7 +206,7 @@ class Server:
         with torch.cuda.stream(a.stream):
             a.plan = None
             p = ShotPlan(req.order, req.nt, req.h, req.dt, [(req.nx, req.ny)], shots, a.K, self.device,
-                         graph_steps=self.graph_steps)
+                         graph_steps=self.graph_steps, forward_only=req.kind == "forward")
             p.models[0].load(a.vel)
             p.forward()
             if req.kind == "gradient":
@@ -242,6 +246,9 @@ class Server:
             else:
                 a.res.frd = a.frd
             a.res.finished = time.perf_counter()
+            if self.verbose:
+                print("serving: request %d (%s) done, latency %.1f ms" % (req.rid, req.kind, 1e3 * a.res.latency),
+                      file=sys.stderr, flush=True)
             self.in_use -= a.cost
             self.free_slots.append(a.slot)
             self.free_slots.sort()
@@ -349,14 +356,14 @@ def main(argv=None):
     dev = torch.device("cuda:0")
     # capacity probe: serve the stream back to back (all arrivals at t=0)
     probe = make_stream(args.requests, 1.0, seed=args.seed, grids=grids, nt_range=ntr, shots_range=shr)
-    srv = Server(dev, slots=args.slots)
+    srv = Server(dev, slots=args.slots, verbose=True)
     t0 = time.perf_counter()
     srv.run(probe, realtime=False)
     cap_s = time.perf_counter() - t0
     capacity = args.requests / cap_s
     rate = args.rate or 0.8 * capacity
     reqs = make_stream(args.requests, rate, seed=args.seed, grids=grids, nt_range=ntr, shots_range=shr)
-    srv = Server(dev, slots=args.slots)
+    srv = Server(dev, slots=args.slots, verbose=True)
     t0 = time.perf_counter()
     res = srv.run(reqs, realtime=True)
     wall = time.perf_counter() - t0

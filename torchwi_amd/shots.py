@@ -19,15 +19,20 @@ from . import _lib
 
 ALIGN = 32  # row stride alignment in floats (128 B)
 GRAPH_STEPS = 100  # time steps per captured hipGraph block (0: direct launches)
+TIME_BLOCK = 1     # time steps per launch of the step kernel (2..4: temporal blocking)
+RING = 4           # fields per wavefield ring (3 rotate for one-step launches; 2 pairs for blocks)
 
 
 def round_up(n, m):
     return (n + m - 1) // m * m
 
 
-def default_ckpt_interval(nt):
-    """About sqrt(nt) steps per checkpoint segment."""
-    return max(2, int(math.ceil(math.sqrt(nt))))
+def default_ckpt_interval(nt, multiple=1):
+    """About sqrt(nt) steps per checkpoint segment, rounded up to a multiple
+    of the kernel's time block."""
+    k = max(2, int(math.ceil(math.sqrt(nt))))
+    m = max(1, int(multiple))
+    return (k + m - 1) // m * m
 
 
 class ModelGrid:
@@ -57,10 +62,20 @@ class ModelGrid:
             self.vel[:, :self.dimx].copy_(vp)
 
 
-def plan_bytes(models, shots, nt, ckpt_every):
+def plan_bytes(models, shots, nt, ckpt_every, forward_only=False):
     """Device bytes of a ShotPlan (models: list of (nx, ny, order);
     shots: list of dicts with 'model' and 'nrec')."""
     K = int(ckpt_every)
+    if forward_only:
+        tot = 0
+        fs = []
+        for nx, ny, order in models:
+            f = (ny + order) * round_up(nx + order, ALIGN)
+            fs.append(f)
+            tot += 3 * f
+        for q in shots:
+            tot += fs[q["model"]] * RING + nt * q["nrec"]
+        return 4 * tot
     nseg = 0 if K == 0 else (nt + K - 1) // K
     slots = nt if K == 0 else K
     fs = []
@@ -71,7 +86,7 @@ def plan_bytes(models, shots, nt, ckpt_every):
         tot += 3 * f
     for q in shots:
         f = fs[q["model"]]
-        tot += f * (3 + (3 if K else 0) + slots + 2 * nseg + 1) + 2 * nt * q["nrec"]
+        tot += f * (RING + (RING if K else 0) + slots + 2 * nseg + 1) + 2 * nt * q["nrec"]
     return 4 * tot
 
 
@@ -86,10 +101,13 @@ class ShotPlan:
     ckpt_every: 0 = full virtual-source storage, K > 0 = checkpoints.
     graph_steps: time steps per replayed hipGraph block (0: one direct launch
             per step).
+    time_block: time steps advanced per launch by the temporally blocked
+            kernel (2..4; 1: one step per launch).  ckpt_every must then be
+            a multiple of it.
     """
 
     def __init__(self, order, nt, h, dt, models, shots, ckpt_every, device, tile_rows=0,
-                 graph_steps=GRAPH_STEPS):
+                 graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK, forward_only=False):
         self.order, self.nt, self.h, self.dt = order, int(nt), float(h), float(dt)
         self.device = torch.device(device)
         self.K = int(ckpt_every)
@@ -113,17 +131,25 @@ class ShotPlan:
             racc += self.nt * nrec[s]
         nf = acc
         e = lambda n: torch.empty(n, dtype=torch.float32, device=dev)  # noqa: E731
-        self.u = torch.zeros(3 * nf, dtype=torch.float32, device=dev)
-        self.lam = torch.zeros(3 * nf, dtype=torch.float32, device=dev) if self.K else self.u
-        self.virt = e(slots * nf)
-        self.ckpt = e(2 * self.nseg * nf) if self.K else None
-        self.grad = torch.zeros(nf, dtype=torch.float32, device=dev)
+        self.forward_only = bool(forward_only)
+        if self.forward_only and self.K:
+            raise ValueError("forward_only plans take ckpt_every = 0")
+        self.u = torch.zeros(RING * nf, dtype=torch.float32, device=dev)
         self.frd = torch.zeros(racc, dtype=torch.float32, device=dev)
-        self.resid = torch.zeros(racc, dtype=torch.float32, device=dev)
+        if self.forward_only:
+            self.lam = self.virt = self.ckpt = self.grad = self.resid = None
+        else:
+            self.lam = torch.zeros(RING * nf, dtype=torch.float32, device=dev) if self.K else self.u
+            self.virt = e(slots * nf)
+            self.ckpt = e(2 * self.nseg * nf) if self.K else None
+            self.grad = torch.zeros(nf, dtype=torch.float32, device=dev)
+            self.resid = torch.zeros(racc, dtype=torch.float32, device=dev)
         self._wavelets = []
         spec = _lib.TwPlanSpec(order=order, nt=self.nt, h=self.h, dt=self.dt, ckpt_every=self.K,
-                               tile_rows=int(tile_rows), graph_steps=int(graph_steps))
+                               tile_rows=int(tile_rows), graph_steps=int(graph_steps),
+                               time_block=int(time_block), forward_only=int(self.forward_only))
         self.graph_steps = int(graph_steps)
+        self.time_block = int(time_block)
         cs = (_lib.TwShotSpec * S)()
         wcache = {}
         for s, q in enumerate(self.shots):
@@ -145,14 +171,15 @@ class ShotPlan:
             c.rx0, c.nrec = int(q.get("rx0", 0)), nrec[s]
             c.vel, c.vm, c.vc3 = m.vel.data_ptr(), m.vm.data_ptr(), m.vc3.data_ptr()
             c.w = wcache[key].data_ptr()
-            # ring k of shot s lives at arena offset 3*o + k*f: contiguous 3 fields
-            c.u = self.u.data_ptr() + 4 * 3 * o
-            c.lam = self.lam.data_ptr() + 4 * 3 * o
+            # ring field k of shot s lives at arena offset RING*o + k*f
+            c.u = self.u.data_ptr() + 4 * RING * o
             c.frd = self.frd.data_ptr() + 4 * self.roff[s]
-            c.resid = self.resid.data_ptr() + 4 * self.roff[s]
-            c.virt = self.virt.data_ptr() + 4 * slots * o
-            c.ckpt = (self.ckpt.data_ptr() + 4 * 2 * self.nseg * o) if self.K else None
-            c.grad = self.grad.data_ptr() + 4 * o
+            if not self.forward_only:
+                c.lam = self.lam.data_ptr() + 4 * RING * o
+                c.resid = self.resid.data_ptr() + 4 * self.roff[s]
+                c.virt = self.virt.data_ptr() + 4 * slots * o
+                c.ckpt = (self.ckpt.data_ptr() + 4 * 2 * self.nseg * o) if self.K else None
+                c.grad = self.grad.data_ptr() + 4 * o
         self._fs, self._nrec = fs, nrec
         L = _lib.lib()
         nbytes = L.tw_plan_workspace_bytes(ctypes.byref(spec), cs, S)
