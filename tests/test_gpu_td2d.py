@@ -350,3 +350,30 @@ def test_temporal_blocking_bitexact(order, T, nx, ny, nt, K):
     frd_o, _ = O.forward(vpad, c["w"], order, c["dimx"], c["dimy"], c["h"], c["dt"], sx[1], sy[1], ry[1],
                          store_virt=False)
     assert np.array_equal(res[1][0][1].cpu().numpy(), frd_o)
+
+
+def test_forward_only_plan_and_autotune():
+    """A forward-only plan (no virtual-source storage) models the same gathers
+    as a gradient plan and refuses the adjoint; the autotuner returns one of
+    its candidates with a timing for each."""
+    from torchwi_amd import Time2dBatch
+    from torchwi_amd.autotune import CANDIDATES, autotune
+    from torchwi_amd.shots import ShotPlan
+    order = 8
+    c = small_case(order, nx=90, ny=40, nt=120, seed=41)
+    w = torch.from_numpy(c["w"])
+    vel = torch.from_numpy(c["vel"]).to(DEV)
+    shots = [dict(model=0, sx=(i + 0.5) * c["h"], sy=2.5 * c["h"], ry=2.5 * c["h"], rx0=0, nrec=c["nx"], w=w)
+             for i in (10, 50)]
+    p = ShotPlan(order, c["nt"], c["h"], c["dt"], [(c["nx"], c["ny"])], shots, 0, DEV, forward_only=True)
+    p.models[0].load(vel)
+    p.forward()
+    m = Time2dBatch(c["nx"], c["ny"], c["h"], w, c["dt"], order, DEV, ckpt_every=0)
+    frd = m.modeling(vel, [q["sx"] for q in shots], [q["sy"] for q in shots], [q["ry"] for q in shots])
+    assert torch.equal(p.frd_all(), frd)
+    with pytest.raises(RuntimeError, match="forward_only"):
+        p.backward()
+    best, times = autotune(order, c["h"], c["dt"], [(c["nx"], c["ny"])],
+                           [dict(q, w=None) for q in shots], DEV, nt_probe=30)
+    assert best in [dict(x) for x in CANDIDATES]
+    assert len(times) == len(CANDIDATES)

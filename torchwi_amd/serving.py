@@ -361,6 +361,8 @@ def main(argv=None):
     srv.run(probe, realtime=False)
     cap_s = time.perf_counter() - t0
     capacity = args.requests / cap_s
+    del srv, probe
+    torch.cuda.empty_cache()
     rate = args.rate or 0.8 * capacity
     reqs = make_stream(args.requests, rate, seed=args.seed, grids=grids, nt_range=ntr, shots_range=shr)
     srv = Server(dev, slots=args.slots, verbose=True)

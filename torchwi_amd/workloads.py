@@ -115,3 +115,28 @@ def config3(order=8, nshots=64, nt=8000):
         vel=to_user(v), w=ricker(6.0, dt, nt).astype(np.float32),
         sx=[float(cell_pos(int(x), h)) for x in xs], sy=[float(cell_pos(2, h))] * nshots,
         ry=[float(cell_pos(2, h))] * nshots)
+
+
+def config4(order=8, nshots=256, nt=6000, seed=4, margin=40):
+    """Batched uneven shots: 600x4000 model, h=10 m, dt=1 ms, nt=6000, smooth
+    random layered 1500-4000 m/s; 256 shots with receiver windows of
+    U{400..2000} cells (seeded), each shot's sub-grid its window +-40 cells
+    (clipped to the model), the shot at the window centre, receivers on the
+    window's columns at z=2."""
+    nz, nx, h, dt = 600, 4000, 10.0, 1e-3
+    v = smooth_layered(nz, nx, seed=3)
+    rng = np.random.default_rng(seed)
+    widths = rng.integers(400, 2001, nshots)
+    centres = (np.arange(nshots) + 0.5) * nx / nshots
+    wins = []
+    for k in range(nshots):
+        wdt = int(widths[k])
+        c = int(centres[k])
+        r0 = max(0, min(nx - wdt, c - wdt // 2))
+        x0 = max(0, r0 - margin)
+        x1 = min(nx, r0 + wdt + margin)
+        src = min(max(c, r0), r0 + wdt - 1)
+        wins.append(dict(x0=x0, x1=x1, sx=float(cell_pos(src - x0, h)), sy=float(cell_pos(2, h)),
+                         ry=float(cell_pos(2, h)), rx0=r0 - x0, nrec=wdt))
+    return dict(nx=nx, ny=nz, h=h, dt=dt, order=order, nt=nt, vel=to_user(v),
+                w=ricker(10.0, dt, nt).astype(np.float32), windows=wins)
