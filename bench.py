@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-timing", action="store_true", help="skip per-launch HIP-event timing")
     ap.add_argument("--time-block", type=int, default=1,
                     help="time steps per launch (2..4: temporally blocked kernel)")
+    ap.add_argument("--tile-rows", type=int, default=0,
+                    help="tile height (one step per launch: 32 or 64; blocked kernel: rows per chunk); 0 automatic")
     ap.add_argument("--graph-steps", type=int, default=100,
                     help="time steps per captured hipGraph block (0: one direct launch per step)")
     ap.add_argument("--compare", type=int, default=1,
@@ -82,7 +84,7 @@ def cpu_baseline():
     from torchwi_amd import workloads as W
     cores = len(os.sched_getaffinity(0))
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    c = W.config1()
+    c = W.three_layer_100x200()
     order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
     dimx, dimy = nx + order, ny + order
     vpad = O.pad_vel(c["vel"], order)
@@ -109,14 +111,14 @@ def cpu_baseline():
                       "forward with full storage + adjoint gradient; median of 5 runs (%.3f s each)" % t}
 
 
-def run_batched(c, K, steps, warmup, timing, graph_steps, dev, time_block=1):
+def run_batched(c, K, steps, warmup, timing, graph_steps, dev, time_block=1, tile_rows=0):
     """Time `steps` misfit gradients of the batched engine; returns stats."""
     import torch
     from torchwi_amd import Time2dBatch
     order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
     S = len(c["sx"])
     model = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K,
-                        graph_steps=graph_steps, time_block=time_block)
+                        graph_steps=graph_steps, time_block=time_block, tile_rows=tile_rows)
     vel = torch.from_numpy(c["vel"]).to(dev)
     rng = np.random.default_rng(7)
     vtrue = torch.from_numpy(c["vel"] * (1.0 + 0.02 * rng.standard_normal(c["vel"].shape)).astype(np.float32)).to(dev)
@@ -183,7 +185,7 @@ def run_plain(c, nshots, dev):
 def load_traffic(kernel_key):
     """HBM bytes per launch of a step kernel at this workload, from the
     calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs kept in profiles/."""
-    path = os.path.join(ROOT, "profiles", "traffic_k_vstep_config2.json")
+    path = os.path.join(ROOT, "profiles", "traffic_k_vstep_384x1024_32shots.json")
     try:
         with open(path) as f:
             return json.load(f)["kernels"].get(kernel_key)
@@ -204,12 +206,12 @@ def main():
 
     dev = torch.device("cuda:0")
     probe = copy_probe(args.copy_probe) if args.copy_probe else None
-    c = W.config2(nshots=args.nshots, nt=args.nt)
+    c = W.dipping_384x1024(nshots=args.nshots, nt=args.nt)
     order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
     tb = args.time_block
     kck = args.ckpt_every or default_ckpt_interval(nt, tb)
     K = {"full": 0, "ckpt": kck, "auto": "auto"}[args.mode]
-    r = run_batched(c, K, args.steps, args.warmup, not args.no_timing, args.graph_steps, dev, tb)
+    r = run_batched(c, K, args.steps, args.warmup, not args.no_timing, args.graph_steps, dev, tb, args.tile_rows)
     S, kstate, value, elapsed, lat = r["S"], r["K"], r["value"], r["elapsed"], r["lat"]
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
@@ -222,7 +224,7 @@ def main():
                                % (S, ny, nx, r["dimy"], r["dimx"], h, dt, nt, order,
                                   "full virtual-source storage" if kstate == 0 else "checkpoint every %d steps + recompute" % kstate),
                    "shots": S, "nz": ny, "nx": nx, "nt": nt, "order": order, "ckpt_every": kstate,
-                   "graph_steps": args.graph_steps, "time_block": tb},
+                   "graph_steps": args.graph_steps, "time_block": tb, "tile_rows": args.tile_rows},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "gradient_shots_per_s": round(r["shots_per_s"], 3),
         "effective_hbm_frac": round(value * 1e9 * (BYTES_GRAD_STEP / 2.0) / (HBM_PEAK_GBS * 1e9), 4),
@@ -248,7 +250,7 @@ def main():
                            "traffic": tr, "bytes_per_cell": b, "cells_per_launch": cells_launch}
     if args.compare:
         if kstate == 0:
-            rc = run_batched(c, kck, 1, 0, False, args.graph_steps, dev, tb)
+            rc = run_batched(c, kck, 1, 0, False, args.graph_steps, dev, tb, args.tile_rows)
             out["checkpointed"] = {"ckpt_every": rc["K"], "value": round(rc["value"], 3),
                                    "ms_per_step": round(rc["elapsed"] * 1e3, 3),
                                    "gradient_shots_per_s": round(rc["shots_per_s"], 3),

@@ -7,7 +7,7 @@ inputs and outputs are stored (float32 .npz):
 
   td2d_o{2,4,8}.npz  seeded random-velocity case on a 37x23 model, nt=160:
                      vel, w, positions, resid -> frd, grad, and two virt slices
-  config1_o8.npz     the 100x200 three-layer model behind a 20-cell pad,
+  three_layer_100x200_o8.npz     the 100x200 three-layer model behind a 20-cell pad,
                      nt=1000, order 8: forward gather and the L2-misfit
                      gradient against the gather of the +5% blob model
                      (TimeFunction.py crop/transposition applied to grad)
@@ -41,8 +41,8 @@ def small(order):
                 virt_slices=virt[slices])
 
 
-def config1():
-    c = W.config1()
+def three_layer_100x200():
+    c = W.three_layer_100x200()
     order, nx, ny = c["order"], c["nx"], c["ny"]
     dimx, dimy = nx + order, ny + order
     vpad = O.pad_vel(c["vel"], order)
@@ -60,7 +60,7 @@ def main():
         sys.exit("oracle/_ref/libtd2d_ref.so is not built: run `make -C oracle ref`")
     for order in (2, 4, 8):
         np.savez_compressed(os.path.join(HERE, "td2d_o%d.npz" % order), **small(order))
-    np.savez_compressed(os.path.join(HERE, "config1_o8.npz"), **config1())
+    np.savez_compressed(os.path.join(HERE, "three_layer_100x200_o8.npz"), **three_layer_100x200())
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -178,12 +178,12 @@ def test_batch_matches_single_and_checkpoint_matches_full(order):
         assert np.array_equal(got, grad_o)
 
 
-def test_config1_forward_gradient_bitexact():
+def test_three_layer_100x200_forward_gradient_bitexact():
     """The reference's CPU-runnable case: 100x200 model behind a 20-cell pad,
     nt=1000, order 8, gradient of the L2 misfit against the blob model."""
     from torchwi_amd import workloads as W
     import torchwi_amd as torchwi
-    c = W.config1()
+    c = W.three_layer_100x200()
     order, nx, ny, nt = c["order"], c["nx"], c["ny"], c["nt"]
     dimx, dimy = nx + order, ny + order
     vpad = O.pad_vel(c["vel"], order)
@@ -204,7 +204,7 @@ def test_config1_forward_gradient_bitexact():
     loss = 0.5 * torch.sum((frd - torch.from_numpy(d_obs).to(DEV)) ** 2)
     loss.backward()
     f, g = frd.detach().cpu().numpy(), vel.grad.cpu().numpy()
-    print(f"config1: rel-L2 vs oracle frd {rel_l2(f, frd_o):.2e} grad {rel_l2(g, grad_o):.2e}; "
+    print(f"three-layer 100x200: rel-L2 vs oracle frd {rel_l2(f, frd_o):.2e} grad {rel_l2(g, grad_o):.2e}; "
           f"vs FP64 frd {rel_l2(f, frd64):.2e} grad {rel_l2(g, grad64):.2e}")
     assert np.array_equal(f, frd_o)
     assert np.array_equal(g, grad_o)
@@ -321,24 +321,28 @@ def test_windows_match_oracle_on_cropped_models():
     assert rel_l2(g.cpu().numpy(), want) < 1e-6
 
 
-@pytest.mark.parametrize("order,T,nx,ny,nt,K", [
-    (8, 3, 150, 70, 100, 0), (8, 3, 150, 70, 101, 12), (8, 2, 61, 130, 77, 8), (8, 4, 200, 64, 90, 16),
-    (4, 3, 131, 57, 95, 9), (2, 3, 67, 29, 80, 0), (2, 4, 70, 150, 60, 8), (8, 3, 20, 12, 40, 6)])
-def test_temporal_blocking_bitexact(order, T, nx, ny, nt, K):
-    """The temporally blocked kernel (T steps per launch, redundant halo)
-    gives the same gathers and per-shot gradients as one step per launch,
-    which equal the oracle; with checkpoints (K a multiple of T) too."""
+@pytest.mark.parametrize("order,T,nx,ny,nt,K,rows", [
+    (8, 3, 150, 70, 100, 0, 0), (8, 3, 150, 70, 101, 12, 0), (8, 2, 61, 130, 77, 8, 0),
+    (8, 4, 200, 64, 90, 16, 0), (4, 3, 131, 57, 95, 9, 0), (2, 3, 67, 29, 80, 0, 0),
+    (2, 4, 70, 150, 60, 8, 0), (8, 3, 20, 12, 40, 6, 0), (8, 4, 300, 200, 63, 0, 1000),
+    (8, 4, 141, 203, 70, 8, 37), (8, 2, 250, 90, 51, 0, 16), (4, 4, 77, 120, 45, 12, 50)])
+def test_temporal_blocking_bitexact(order, T, nx, ny, nt, K, rows):
+    """The temporally blocked kernel (T steps per launch over column strips
+    and row chunks with redundant halos) gives the same gathers and per-shot
+    gradients as one step per launch, which equal the oracle; with
+    checkpoints (K a multiple of T) too; rows: chunk height (0 automatic)."""
     from torchwi_amd import Time2dBatch
     c = small_case(order, nx=nx, ny=ny, nt=nt, seed=nx * 7 + T)
     sx = [(i + 0.5) * c["h"] for i in (1, nx // 2, nx - 2)]
     sy = [(i + 0.5) * c["h"] for i in (0, ny // 3, ny - 1)]
     ry = [(i + 0.5) * c["h"] for i in (1, 0, ny // 2)]
-    print(f"case order={order} T={T} nx={nx} ny={ny} nt={nt} K={K}", flush=True)
+    print(f"case order={order} T={T} nx={nx} ny={ny} nt={nt} K={K} rows={rows}", flush=True)
     vel = torch.from_numpy(c["vel"]).to(DEV)
     w = torch.from_numpy(c["w"])
     res = []
     for tb in (1, T):
-        m = Time2dBatch(nx, ny, c["h"], w, c["dt"], order, DEV, ckpt_every=K, time_block=tb)
+        m = Time2dBatch(nx, ny, c["h"], w, c["dt"], order, DEV, ckpt_every=K, time_block=tb,
+                        tile_rows=rows if tb > 1 else 0)
         frd = m.modeling(vel, sx, sy, ry).clone()
         loss, g = m.misfit_gradient(vel, sx, sy, ry, 0.6 * frd)
         res.append((frd, g.clone(), [m.plan.grad_of(s).clone() for s in range(3)], float(loss)))

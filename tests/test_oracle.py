@@ -46,8 +46,8 @@ def test_oracle_matches_reference_golden(order):
     assert np.array_equal(fo, g["frd"])
 
 
-def test_oracle_matches_reference_golden_config1():
-    g = _load("config1_o8.npz")
+def test_oracle_matches_reference_golden_three_layer_100x200():
+    g = _load("three_layer_100x200_o8.npz")
     order = int(g["order"])
     nx, ny = g["vel"].shape
     dimx, dimy = nx + order, ny + order

@@ -49,6 +49,7 @@ enum : unsigned {
 struct ShotDesc {
     int dimx, dimy, ld;
     int tiles_x, tile0;
+    int mstrips, mchunks;     // temporally blocked kernel: column strips x row chunks
     int rrow, rx0, nrec;      // receiver row / window in padded coordinates
     long long isrc;           // linear (ld-strided) index of the source cell
     long long fstride;        // floats between consecutive fields of a ring
@@ -253,7 +254,7 @@ __global__ __launch_bounds__(NTHR) void k_step_single(ShotDesc d, StepArgs a)
 }  // namespace tw
 
 #include "td2d_vec.hpp"
-#include "td2d_temporal.hpp"
+#include "td2d_march.hpp"
 
 namespace tw {
 
@@ -658,6 +659,42 @@ int choose_ryv(const std::vector<int>& dimys, int ne)
     return pick;
 }
 
+// Work list of the temporally blocked kernel: every shot's grid split into
+// column strips of at most 4*MC4 cells (as even as float4 columns allow) and
+// row chunks.  chunk_rows > 0 sets the chunk height; 0 picks the fewest
+// chunks that give MWG_TARGET workgroups.  Dealt model by model, chunk- and
+// strip-major and shot-minor, so an XCD's band of workgroups shares model
+// rows.
+int march_tiles(Plan& p, int chunk_rows)
+{
+    if (chunk_rows < 0 || (chunk_rows > 0 && chunk_rows < 16))
+        return fail(TW_EINVAL, "tile_rows for time_block > 1 must be 0 or >= 16 (got %d)", chunk_rows);
+    long long strips = 0;
+    for (const ShotDesc& d : p.host) strips += ((d.dimx + 3) / 4 + MC4 - 1) / MC4;
+    const int want = (int)((MWG_TARGET + strips - 1) / strips);
+    for (const ModelRef& m : p.models) {
+        const int dimx4 = (m.dimx + 3) / 4;
+        const int ns = (dimx4 + MC4 - 1) / MC4;
+        int nch = chunk_rows > 0 ? (m.dimy + chunk_rows - 1) / chunk_rows : want;
+        const int most = m.dimy / 16 > 1 ? m.dimy / 16 : 1;   // chunks of >= 16 rows
+        if (nch > most) nch = most;
+        if (nch < 1) nch = 1;
+        std::vector<int> group;
+        for (int s = 0; s < p.nshots; ++s)
+            if (p.host[s].vel == m.vel) {
+                group.push_back(s);
+                p.host[s].mstrips = ns;
+                p.host[s].mchunks = nch;
+            }
+        for (int c = 0; c < nch; ++c)
+            for (int j = 0; j < ns; ++j)
+                for (int s : group) p.tiles.push_back(make_int2(s, (c << 16) | j));
+    }
+    p.ntiles = (int)p.tiles.size();
+    p.per_xcd = (p.ntiles + NXCD - 1) / NXCD;
+    return TW_OK;
+}
+
 int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan& p)
 {
     if (!spec || !shots || nshots <= 0) return fail(TW_EINVAL, "empty shot list");
@@ -746,10 +783,11 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
             dimys.push_back(q.dimy);
         }
     }
+    if (p.tb > 1) return march_tiles(p, spec->tile_rows);
     if (spec->tile_rows == 32 || spec->tile_rows == 64) p.ryv = spec->tile_rows / 16;
     else if (spec->tile_rows == 0) p.ryv = choose_ryv(dimys, ne);
     else return fail(TW_EINVAL, "tile_rows must be 0, 32 or 64 (got %d)", spec->tile_rows);
-    const int ty = p.tb > 1 ? TCY : 16 * p.ryv;
+    const int ty = 16 * p.ryv;
     // Deal tiles model by model; within a model spatial-tile-major and
     // shot-minor, so consecutive workgroups of an XCD share model rows.
     for (const ModelRef& m : p.models) {
@@ -910,7 +948,7 @@ int replay(Plan& p, Graph& gr, int kind, int steps, hipStream_t s)
 template <int NE, unsigned FLAGS, int T>
 void launch_t_nft(const Plan& p, const TArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_tstep<NE, FLAGS, T>), dim3(p.per_xcd * NXCD), dim3(TTHR), 0, s, p.d_desc,
+    hipLaunchKernelGGL((k_mstep<NE, FLAGS, T>), dim3(p.per_xcd * NXCD), dim3(MTHR), 0, s, p.d_desc,
                        p.d_tiles, a);
 }
 

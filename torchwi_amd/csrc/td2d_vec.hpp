@@ -49,6 +49,14 @@ __device__ __forceinline__ void gst4_nt(float* p, float4 v)
     f4v t = {v.x, v.y, v.z, v.w};
     __builtin_nontemporal_store(t, (gf4v*)p);
 }
+// Scalar global-address-space accesses (gathers, residuals, wavelet): a
+// plain store through a descriptor pointer compiles to a FLAT store, and a
+// pending FLAT operation makes every later wait on the memory counters a wait
+// for all outstanding operations.
+typedef __attribute__((address_space(1))) float gf1;
+__device__ __forceinline__ float gld1(const float* p) { return *(const gf1*)p; }
+__device__ __forceinline__ void gst1(float* p, float v) { *(gf1*)p = v; }
+
 __device__ __forceinline__ float4 sel4(bool c, float4 a)
 {
     return make_float4(c ? a.x : 0.f, c ? a.y : 0.f, c ? a.z : 0.f, c ? a.w : 0.f);
@@ -149,11 +157,11 @@ __global__ __launch_bounds__(VTHR) void k_vstep(const ShotDesc* __restrict__ des
                 if (gy < NE) {
                     val = 0.f;
                 } else if (gy >= dimy - NE) {
-                    val = keys_update(um0, uo0, um[ij - ld], Tc[j - W], Tc[j - 2 * W], d.vel[ij], d.h2, d.dt2, d.hdt);
+                    val = keys_update(um0, uo0, um[ij - ld], Tc[j - W], Tc[j - 2 * W], gld1(d.vel + ij), d.h2, d.dt2, d.hdt);
                 } else if (x < NE) {
-                    val = keys_update(um0, uo0, um[ij + 1], Tc[j + 1], Tc[j + 2], d.vel[ij], d.h2, d.dt2, d.hdt);
+                    val = keys_update(um0, uo0, um[ij + 1], Tc[j + 1], Tc[j + 2], gld1(d.vel + ij), d.h2, d.dt2, d.hdt);
                 } else if (x >= dimx - NE) {
-                    val = keys_update(um0, uo0, um[ij - 1], Tc[j - 1], Tc[j - 2], d.vel[ij], d.h2, d.dt2, d.hdt);
+                    val = keys_update(um0, uo0, um[ij - 1], Tc[j - 1], Tc[j - 2], gld1(d.vel + ij), d.h2, d.dt2, d.hdt);
                 } else {
                     val = interior_update<NE>(Tc + j, W, um0, f4get(pm[r], j));
                 }
@@ -162,7 +170,7 @@ __global__ __launch_bounds__(VTHR) void k_vstep(const ShotDesc* __restrict__ des
         }
         if constexpr ((FLAGS & F_SRC) != 0) {
             if (d.isrc >= i && d.isrc < i + 4) {
-                const float add = inject_term(d.w[a.it], d.vel[d.isrc], d.dt2);
+                const float add = inject_term(gld1(d.w + a.it), gld1(d.vel + d.isrc), d.dt2);
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                     if (d.isrc == i + j) f4set(un, j, f4get(un, j) + add);
@@ -175,8 +183,8 @@ __global__ __launch_bounds__(VTHR) void k_vstep(const ShotDesc* __restrict__ des
                 for (int j = 0; j < 4; ++j) {
                     const int x = gx + j;
                     if (x >= d.rx0 && x < d.rx0 + d.nrec)
-                        f4set(un, j, f4get(un, j) + inject_term(d.resid[(long long)a.it * d.nrec + (x - d.rx0)],
-                                                                d.vel[i + j], d.dt2));
+                        f4set(un, j, f4get(un, j) + inject_term(gld1(d.resid + (long long)a.it * d.nrec + (x - d.rx0)),
+                                                                gld1(d.vel + i + j), d.dt2));
                 }
             }
         }
@@ -193,7 +201,7 @@ __global__ __launch_bounds__(VTHR) void k_vstep(const ShotDesc* __restrict__ des
                 for (int j = 0; j < 4; ++j) {
                     const int x = gx + j;
                     if (x >= d.rx0 && x < d.rx0 + d.nrec)
-                        d.frd[(long long)a.it * d.nrec + (x - d.rx0)] = f4get(un, j);
+                        gst1(d.frd + (long long)a.it * d.nrec + (x - d.rx0), f4get(un, j));
                 }
             }
         }

@@ -69,7 +69,7 @@ def main(argv=None):
     ap.add_argument("--no-full", action="store_true", help="skip the full-storage comparison")
     args = ap.parse_args(argv)
     dev = torch.device("cuda:0")
-    c = W.config3(nshots=args.shots, nt=args.nt)
+    c = W.dipping_1000x3000(nshots=args.shots, nt=args.nt)
     if (args.nz, args.nx) != (1000, 3000):
         v = W.dipping_layers(args.nz, args.nx, seed=2)
         xs = (np.arange(args.shots) + 0.5) * args.nx / args.shots

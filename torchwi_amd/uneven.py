@@ -1,4 +1,4 @@
-"""Batched uneven shots in one plan (config 4), with tile-shape autotuning.
+"""Batched uneven shots in one plan (256 receiver windows of a 600 x 4000 model), with tile-shape autotuning.
 
     python -m torchwi_amd.uneven [--shots 256] [--nt 6000]
 
@@ -28,7 +28,7 @@ def main(argv=None):
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args(argv)
     dev = torch.device("cuda:0")
-    c = W.config4(nshots=args.shots, nt=args.nt)
+    c = W.uneven_600x4000(nshots=args.shots, nt=args.nt)
     order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
     wins = c["windows"]
     out = {"workload": "%d shots on windows of a %dx%d model, h=%g m, dt=%g s, nt=%d, order %d, "

@@ -69,7 +69,7 @@ def to_user(v_zx):
     return np.ascontiguousarray(np.asarray(v_zx, dtype=np.float32).T)
 
 
-def config1(order=8, pad=20, nt=1000):
+def three_layer_100x200(order=8, pad=20, nt=1000):
     """Plain reference-path case: 100x200 model, h=10 m, dt=1 ms, nt=1000,
     Keys ABC behind a pad-cell edge-replicated border (left, right, bottom;
     the top is the free surface), source (z=2, x=100), receivers at z=2 on
@@ -90,7 +90,7 @@ def config1(order=8, pad=20, nt=1000):
         crop=(pad, pad + nx))
 
 
-def config2(order=8, nshots=32, nt=4000):
+def dipping_384x1024(order=8, nshots=32, nt=4000):
     """Plain HIP form: 384x1024 model, h=12.5 m, dt=1 ms, nt=4000, 8 dipping
     layers 1500-4500 m/s with +-5% correlated perturbation (corr. 20 cells),
     32 ricker(8) shots evenly spaced at z=2, 1024 receivers at z=2."""
@@ -104,7 +104,7 @@ def config2(order=8, nshots=32, nt=4000):
         ry=[float(cell_pos(2, h))] * nshots)
 
 
-def config3(order=8, nshots=64, nt=8000):
+def dipping_1000x3000(order=8, nshots=64, nt=8000):
     """Differentiation at scale: 1000x3000, h=10 m, dt=0.8 ms, nt=8000,
     64 ricker(6) shots, 3000 receivers, dipping-layer family."""
     nz, nx, h, dt = 1000, 3000, 10.0, 8e-4
@@ -117,7 +117,7 @@ def config3(order=8, nshots=64, nt=8000):
         ry=[float(cell_pos(2, h))] * nshots)
 
 
-def config4(order=8, nshots=256, nt=6000, seed=4, margin=40):
+def uneven_600x4000(order=8, nshots=256, nt=6000, seed=4, margin=40):
     """Batched uneven shots: 600x4000 model, h=10 m, dt=1 ms, nt=6000, smooth
     random layered 1500-4000 m/s; 256 shots with receiver windows of
     U{400..2000} cells (seeded), each shot's sub-grid its window +-40 cells
