@@ -325,7 +325,9 @@ def test_windows_match_oracle_on_cropped_models():
     (8, 3, 150, 70, 100, 0, 0), (8, 3, 150, 70, 101, 12, 0), (8, 2, 61, 130, 77, 8, 0),
     (8, 4, 200, 64, 90, 16, 0), (4, 3, 131, 57, 95, 9, 0), (2, 3, 67, 29, 80, 0, 0),
     (2, 4, 70, 150, 60, 8, 0), (8, 3, 20, 12, 40, 6, 0), (8, 4, 300, 200, 63, 0, 1000),
-    (8, 4, 141, 203, 70, 8, 37), (8, 2, 250, 90, 51, 0, 16), (4, 4, 77, 120, 45, 12, 50)])
+    (8, 4, 141, 203, 70, 8, 37), (8, 2, 250, 90, 51, 0, 16), (4, 4, 77, 120, 45, 12, 50),
+    (8, 4, 601, 90, 41, 0, 0), (8, 3, 450, 100, 60, 12, 24), (4, 2, 513, 77, 50, 10, 20),
+    (8, 4, 230, 70, 33, 8, 0)])
 def test_temporal_blocking_bitexact(order, T, nx, ny, nt, K, rows):
     """The temporally blocked kernel (T steps per launch over column strips
     and row chunks with redundant halos) gives the same gathers and per-shot

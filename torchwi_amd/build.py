@@ -19,7 +19,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "td2d_hip.hip")
-DEPS = [SRC, os.path.join(HERE, "csrc", "td2d_arith.hpp"), os.path.join(ROOT, "include", "torchwi_hip.h")]
+DEPS = [SRC, os.path.join(ROOT, "include", "torchwi_hip.h")] + [
+    os.path.join(HERE, "csrc", f) for f in ("td2d_arith.hpp", "td2d_vec.hpp", "td2d_pipe.hpp")]
 OUT_DIR = os.path.join(HERE, "lib")
 OUT = os.path.join(OUT_DIR, "libtorchwi_hip.so")
 ARCH = os.environ.get("TORCHWI_AMD_ARCH", "gfx950")

@@ -83,35 +83,29 @@ __device__ __forceinline__ float f4at(const float4& v, int j)
     return j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
 }
 
+// The 4-wide update from values already in registers: up[k-1] / dn[k-1] are
+// the float4s k rows above / below, L, C, R the centre row at x-4, x, x+4.
 template <int NE>
-__device__ __forceinline__ float4 interior_update4(const float* T, int ts, float4 u1, float4 m)
+__device__ __forceinline__ float4 lap_update4(const float4 (&up)[NE], const float4 (&dn)[NE],
+                                              float4 L, float4 C, float4 R, float4 u1, float4 m)
 {
-    const float4 L = *(const float4*)(T - 4);
-    const float4 C = *(const float4*)(T);
-    const float4 Rr = *(const float4*)(T + 4);
-    float4 up[NE + 1], dn[NE + 1];
-#pragma unroll
-    for (int k = 1; k <= NE; ++k) {
-        up[k] = *(const float4*)(T - k * ts);
-        dn[k] = *(const float4*)(T + k * ts);
-    }
     // centre-row value at offset o in [-4, 7]
-    auto row = [&](int o) -> float { return o < 0 ? f4at(L, o + 4) : (o < 4 ? f4at(C, o) : f4at(Rr, o - 4)); };
+    auto row = [&](int o) -> float { return o < 0 ? f4at(L, o + 4) : (o < 4 ? f4at(C, o) : f4at(R, o - 4)); };
     float out[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const float c = f4at(C, j);
         const float um = f4at(u1, j), mm = f4at(m, j);
         if constexpr (NE == 1) {
-            float s = ((row(j - 1) + f4at(up[1], j)) + row(j + 1)) + f4at(dn[1], j);
+            float s = ((row(j - 1) + f4at(up[0], j)) + row(j + 1)) + f4at(dn[0], j);
             float lap = (float)(LapW<1>::w(1) * (double)s + (LapW<1>::w(0) * 2.0) * (double)c);
             out[j] = (2.f * c - um) + mm * lap;
         } else {
-            float s = ((f4at(up[NE], j) + row(j - NE)) + row(j + NE)) + f4at(dn[NE], j);
+            float s = ((f4at(up[NE - 1], j) + row(j - NE)) + row(j + NE)) + f4at(dn[NE - 1], j);
             double acc = LapW<NE>::w(NE) * (double)s;
 #pragma unroll
             for (int k = NE - 1; k >= 1; --k) {
-                s = ((f4at(up[k], j) + row(j - k)) + row(j + k)) + f4at(dn[k], j);
+                s = ((f4at(up[k - 1], j) + row(j - k)) + row(j + k)) + f4at(dn[k - 1], j);
                 acc = acc + LapW<NE>::w(k) * (double)s;
             }
             acc = acc + (LapW<NE>::w(0) * 2.0) * (double)c;
@@ -120,6 +114,18 @@ __device__ __forceinline__ float4 interior_update4(const float* T, int ts, float
         }
     }
     return make_float4(out[0], out[1], out[2], out[3]);
+}
+
+template <int NE>
+__device__ __forceinline__ float4 interior_update4(const float* T, int ts, float4 u1, float4 m)
+{
+    float4 up[NE], dn[NE];
+#pragma unroll
+    for (int k = 1; k <= NE; ++k) {
+        up[k - 1] = *(const float4*)(T - k * ts);
+        dn[k - 1] = *(const float4*)(T + k * ts);
+    }
+    return lap_update4<NE>(up, dn, *(const float4*)(T - 4), *(const float4*)(T), *(const float4*)(T + 4), u1, m);
 }
 
 // Keys update of a boundary cell from its own (um0, uo0) and its first and

@@ -254,7 +254,7 @@ __global__ __launch_bounds__(NTHR) void k_step_single(ShotDesc d, StepArgs a)
 }  // namespace tw
 
 #include "td2d_vec.hpp"
-#include "td2d_march.hpp"
+#include "td2d_pipe.hpp"
 
 namespace tw {
 
@@ -659,23 +659,85 @@ int choose_ryv(const std::vector<int>& dimys, int ne)
     return pick;
 }
 
+// Resident workgroups of the temporally blocked kernel on the device (the
+// least over its step variants), for the work-list sizing below.
+template <int NE, int T>
+int pipe_slots_ne()
+{
+    int dev = 0, cus = 0, best = 1 << 30;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return 0;
+    const void* ks[] = {(const void*)k_pipe<NE, M_FWD, T>, (const void*)k_pipe<NE, M_FWD_VIRT, T>,
+                        (const void*)k_pipe<NE, M_RECOMP, T>, (const void*)k_pipe<NE, M_ADJ, T>};
+    for (const void* k : ks) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 64 * (T + 1), 0) != hipSuccess || n <= 0)
+            return 0;
+        if (n < best) best = n;
+    }
+    return best * cus;
+}
+
+int pipe_slots(int order, int T)
+{
+    const int ne = order / 2;
+    if (T == 2) return ne == 1 ? pipe_slots_ne<1, 2>() : (ne == 2 ? pipe_slots_ne<2, 2>() : pipe_slots_ne<4, 2>());
+    if (T == 3) return ne == 1 ? pipe_slots_ne<1, 3>() : (ne == 2 ? pipe_slots_ne<2, 3>() : pipe_slots_ne<4, 3>());
+    return ne == 1 ? pipe_slots_ne<1, 4>() : (ne == 2 ? pipe_slots_ne<2, 4>() : pipe_slots_ne<4, 4>());
+}
+
 // Work list of the temporally blocked kernel: every shot's grid split into
-// column strips of at most 4*MC4 cells (as even as float4 columns allow) and
-// row chunks.  chunk_rows > 0 sets the chunk height; 0 picks the fewest
-// chunks that give MWG_TARGET workgroups.  Dealt model by model, chunk- and
+// column strips whose core plus halo fits one wavefront (64 float4 columns;
+// cores as even as float4 columns allow) and into row chunks.  chunk_rows > 0
+// sets the chunk height.  0 picks the height that minimises (rounds of
+// resident workgroups) x (iterations of one workgroup, chunk + 9T rows): a
+// workgroup's rows march in sequence, so fewer, taller chunks are better as
+// long as they fill the device once.  Dealt model by model, chunk- and
 // strip-major and shot-minor, so an XCD's band of workgroups shares model
 // rows.
-int march_tiles(Plan& p, int chunk_rows)
+int pipe_tiles(Plan& p, int chunk_rows)
 {
     if (chunk_rows < 0 || (chunk_rows > 0 && chunk_rows < 16))
         return fail(TW_EINVAL, "tile_rows for time_block > 1 must be 0 or >= 16 (got %d)", chunk_rows);
-    long long strips = 0;
-    for (const ShotDesc& d : p.host) strips += ((d.dimx + 3) / 4 + MC4 - 1) / MC4;
-    const int want = (int)((MWG_TARGET + strips - 1) / strips);
+    const int T = p.tb;
+    auto nstrips = [&](int dimx) {
+        const int dimx4 = (dimx + 3) / 4;
+        if (dimx4 <= PLANES) return 1;
+        int n = 2;
+        while ((dimx4 + n - 1) / n > PLANES - 2 * T) ++n;
+        return n;
+    };
+    int maxdy = 0;
+    for (const ModelRef& m : p.models) maxdy = m.dimy > maxdy ? m.dimy : maxdy;
+    int rows = chunk_rows;
+    if (rows == 0) {
+        const int slots = pipe_slots(p.order, T);
+        if (slots <= 0) return fail(TW_EHIP, "occupancy query of the blocked kernel failed");
+        long long best = -1;
+        for (int n = 1; n <= (maxdy / 16 > 1 ? maxdy / 16 : 1); ++n) {
+            const int r = (maxdy + n - 1) / n;
+            long long tiles = 0;
+            for (const ShotDesc& d : p.host) tiles += (long long)nstrips(d.dimx) * ((d.dimy + r - 1) / r);
+            const long long cost = ((tiles + slots - 1) / slots) * (long long)(r + 9 * T);
+            if (best < 0 || cost < best) {
+                best = cost;
+                rows = r;
+            }
+        }
+    }
     for (const ModelRef& m : p.models) {
         const int dimx4 = (m.dimx + 3) / 4;
-        const int ns = (dimx4 + MC4 - 1) / MC4;
-        int nch = chunk_rows > 0 ? (m.dimy + chunk_rows - 1) / chunk_rows : want;
+        const int ns = nstrips(m.dimx);
+        // every strip but the first starts at least T float4 columns in, and
+        // every strip but the last ends at least T before the grid's end
+        for (int j = 0; j < ns; ++j) {
+            const int a4 = j * dimx4 / ns, b4 = (j + 1) * dimx4 / ns;
+            const int lanes = (b4 - a4) + (j > 0 ? T : 0) + (j < ns - 1 ? T : 0);
+            if ((j > 0 && a4 < T) || (j < ns - 1 && b4 > dimx4 - T) || lanes > PLANES)
+                return fail(TW_EINVAL, "internal: strip %d of %d does not fit (dimx %d)", j, ns, m.dimx);
+        }
+        int nch = (m.dimy + rows - 1) / rows;
         const int most = m.dimy / 16 > 1 ? m.dimy / 16 : 1;   // chunks of >= 16 rows
         if (nch > most) nch = most;
         if (nch < 1) nch = 1;
@@ -783,7 +845,7 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
             dimys.push_back(q.dimy);
         }
     }
-    if (p.tb > 1) return march_tiles(p, spec->tile_rows);
+    if (p.tb > 1) return pipe_tiles(p, spec->tile_rows);
     if (spec->tile_rows == 32 || spec->tile_rows == 64) p.ryv = spec->tile_rows / 16;
     else if (spec->tile_rows == 0) p.ryv = choose_ryv(dimys, ne);
     else return fail(TW_EINVAL, "tile_rows must be 0, 32 or 64 (got %d)", spec->tile_rows);
@@ -948,7 +1010,7 @@ int replay(Plan& p, Graph& gr, int kind, int steps, hipStream_t s)
 template <int NE, unsigned FLAGS, int T>
 void launch_t_nft(const Plan& p, const TArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_mstep<NE, FLAGS, T>), dim3(p.per_xcd * NXCD), dim3(MTHR), 0, s, p.d_desc,
+    hipLaunchKernelGGL((k_pipe<NE, FLAGS, T>), dim3(p.per_xcd * NXCD), dim3(64 * (T + 1)), 0, s, p.d_desc,
                        p.d_tiles, a);
 }
 
@@ -1015,7 +1077,7 @@ int prep_models(const Plan& p, hipStream_t s)
         long long bx = (n + 255) / 256;
         if (bx > 2048) bx = 2048;
         hipLaunchKernelGGL(k_model_prep, dim3((unsigned)bx), dim3(256), 0, s, m.vel, m.vm, m.vc3,
-                           m.dimx, m.dimy, m.ld, m.dtoh2, m.cons);
+                           m.dimx, m.dimy, m.ld, p.order / 2, m.dtoh2, m.cons);
         TW_HIP(hipGetLastError());
     }
     return TW_OK;

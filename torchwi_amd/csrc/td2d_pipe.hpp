@@ -1,0 +1,426 @@
+// td2d_pipe.hpp -- level-pipelined, temporally blocked batched step kernel
+// (included by td2d_hip.hip).
+//
+// One launch advances T consecutive time steps ("levels" 1..T) of every
+// shot.  A workgroup owns a column strip of one shot's padded grid, one
+// wavefront wide (64 float4 columns: a core plus T float4 columns of halo on
+// each side that is not a grid edge), and a chunk of its rows plus 4T rows
+// of halo above and below (again except at grid edges).  It marches down the
+// rows, one row per wave per iteration, with one wave per time level:
+//
+//   wave 0 (loader) streams the two input fields (levels -1 and 0), and in
+//          the adjoint the gradient, from HBM into LDS, PD rows ahead;
+//   wave k (level k) at iteration i computes level k of row r = O + i - 5k.
+//
+// Level k lags level k-1 by five rows (the stencil radius, 4, plus one), so
+// every row it needs of level k-1 was produced at an earlier iteration and one
+// barrier per iteration orders the whole pipeline.  Each level wave keeps the
+// nine rows r-4..r+4 of level k-1 in a ten-slot register window (the row loop
+// is unrolled ten times, so every window slot is a fixed register); the x
+// neighbours of the centre row come from the adjacent lanes through DPP
+// wavefront shifts.  LDS carries only what moves between levels: the newest
+// row of each level (two slots), the rows of level k-2 that the next level
+// uses as its u_prev (read five and six iterations later: seven slots), and
+// in the adjoint the gradient sum after each level (six slots), so the
+// gradient is read and written once per launch and accumulated in time order.
+//
+// The halo of a level is valid where its dependency cone stays inside the
+// loaded region; it shrinks by one float4 column and four rows per level on
+// sides that are not grid edges, and the core stays valid at level T.  Every
+// cell uses the same arithmetic as a one-step launch (td2d_arith.hpp), so the
+// results are bit-identical.
+#pragma once
+
+namespace tw {
+
+struct TArgs {
+    int t_first;     // time index of level 1
+    int dir;         // +1 forward in time, -1 adjoint
+    int nsteps;      // levels to compute (1..T)
+    int in_pair, out_pair;  // 0: fields (0,1) of the ring, 1: fields (2,3)
+    int vslot_first; // virt slot of level 1 (advances with dir)
+    int adj;         // 1: adjoint ring
+    int K, nseg;     // checkpointing (forward saves)
+    int ntiles, per_xcd;
+};
+
+constexpr int PLANES = 64;  // float4 columns per strip (one wavefront)
+constexpr int PW = 10;      // register window slots (the row loop's unroll)
+constexpr int PQ = 5;       // register prefetch slots (divides PW)
+constexpr int PD = 4;       // prefetch distance in iterations (< PQ)
+constexpr int NPRV = 7;     // LDS slots of level k-2 rows (read 5 and 6 iterations after the write)
+constexpr int NGRD = 6;     // LDS slots of gradient sums (read 5 iterations after the write)
+
+template <int T, unsigned FLAGS>
+struct PLayout {
+    static constexpr int CUR = 0;                          // ring k: 2 rows of level k
+    static constexpr int PRV = CUR + T * 2 * PLANES;       // ring k: NPRV rows of level k-1
+    static constexpr int GRD = PRV + T * NPRV * PLANES;    // ring k: NGRD rows of gradient sums
+    static constexpr int KS = GRD + ((FLAGS & F_GRAD) != 0 ? T * NGRD * PLANES : 0);  // per level: Keys scratch
+    static constexpr int SIZE = KS + T * 32;
+};
+
+template <int N>
+struct IC {
+    static constexpr int value = N;
+};
+
+// Wavefront shifts: lane l receives the value of lane l-1 (from_left) or of
+// lane l+1 (from_right); the lanes at the ends receive 0.
+__device__ __forceinline__ float dpp_from_left(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float dpp_from_right(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float4 left4(float4 v)
+{
+    return make_float4(dpp_from_left(v.x), dpp_from_left(v.y), dpp_from_left(v.z), dpp_from_left(v.w));
+}
+__device__ __forceinline__ float4 right4(float4 v)
+{
+    return make_float4(dpp_from_right(v.x), dpp_from_right(v.y), dpp_from_right(v.z), dpp_from_right(v.w));
+}
+// Value at offset o in [-4, 7] of a row given as the float4s at x-4, x, x+4.
+__device__ __forceinline__ float at3(const float4& a, const float4& b, const float4& c, int o)
+{
+    return o < 0 ? f4at(a, o + 4) : (o < 4 ? f4at(b, o) : f4at(c, o - 4));
+}
+
+template <int NE, unsigned FLAGS, int T>
+__global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restrict__ descs,
+                                                       const int2* __restrict__ tiles, TArgs a)
+{
+    using LY = PLayout<T, FLAGS>;
+    __shared__ float4 S[LY::SIZE];
+
+    const int Lt = logical_tile(blockIdx.x, a.per_xcd);
+    if (Lt >= a.ntiles) return;
+    const int2 e = tiles[Lt];
+    const ShotDesc& d = descs[e.x];
+    const int dimx = d.dimx, dimy = d.dimy;
+    const int ld = d.ld;
+    const int dimx4 = (dimx + 3) >> 2;
+    const int strip = e.y & 0xffff, chunk = e.y >> 16;
+    const int c4a = strip * dimx4 / d.mstrips, c4b = (strip + 1) * dimx4 / d.mstrips;
+    const int ox4 = c4a == 0 ? 0 : c4a - T;          // float4 column of lane 0
+    const int Y0 = chunk * dimy / d.mchunks, Y1 = (chunk + 1) * dimy / d.mchunks;
+    const int O = Y0 - 4 * T > 0 ? Y0 - 4 * T : 0;   // loaded rows [O, E)
+    const int E = Y1 + 4 * T < dimy ? Y1 + 4 * T : dimy;
+    const bool ey0 = O == 0, ey1 = E == dimy;
+    const int kx4 = (dimx - NE) >> 2;                // first float4 column holding a right Keys cell
+    const bool keys_left = ox4 == 0;                 // lane 0 holds the left Keys columns
+    const bool keys_right = ox4 + PLANES - 1 >= kx4; // some lane holds a right Keys column
+
+    const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // 0: loader, else level
+    const int lane = threadIdx.x & 63;
+    const int g4 = ox4 + lane;
+    const int gx = 4 * g4;
+    const bool lin = g4 < dimx4;
+    const bool lcore = g4 >= c4a && g4 < c4b;
+
+    float* const* ring = a.adj ? d.lam : d.u;
+    const float* __restrict__ in_prev = ring[2 * a.in_pair];
+    const float* __restrict__ in_cur = ring[2 * a.in_pair + 1];
+    float* __restrict__ out_prev = ring[2 * a.out_pair];
+    float* __restrict__ out_cur = ring[2 * a.out_pair + 1];
+
+    // rows of this wave's level: [Ok, Ek) (the loader's are [O, E))
+    const int Ok = ey0 ? 0 : O + 4 * k;
+    const int Ek = ey1 ? dimy : E - 4 * k;
+    const bool active = k <= a.nsteps;
+    const int t = a.t_first + a.dir * (k - 1);
+    const long long voff = (long long)(a.vslot_first + a.dir * (k - 1)) * d.fstride;
+
+    // ---- prefetch streams: three register queues shared by all roles
+    //   A: loader u_prev rows;  levels: m (v at Keys cells)
+    //   B: loader u_cur rows;   levels: c3 (virtual-source store) or virt (adjoint)
+    //   C: loader gradient rows (adjoint)
+    const float* pA;
+    const float* pB;
+    int aLo, aHi, bLo, bHi;
+    bool aLane, bLane;
+    bool useB;
+    if (k == 0) {
+        pA = in_prev;
+        pB = in_cur;
+        aLo = bLo = O;
+        aHi = bHi = E;
+        aLane = bLane = lin;
+        useB = true;
+    } else {
+        pA = d.vm;
+        aLo = Ok;
+        aHi = Ek;
+        aLane = lin;
+        bLo = Y0 > Ok ? Y0 : Ok;
+        bHi = Y1 < Ek ? Y1 : Ek;
+        bLane = lcore;
+        if constexpr ((FLAGS & F_GRAD) != 0) {
+            // a level past this launch's last step has no virtual-source slot
+            pB = active ? d.virt + voff : d.vc3;
+            useB = active;
+        } else if constexpr ((FLAGS & F_VIRT) != 0) {
+            pB = d.vc3;
+            useB = true;
+        } else {
+            pB = d.vc3;
+            useB = false;
+        }
+    }
+    // Every wave issues every stream's load at every iteration (from cell 0
+    // when the value will not be used), so the in-order count of outstanding
+    // vector-memory operations is the same on every path and the wait for one
+    // queue slot never waits for the slots behind it.
+    float4 qA[PQ], qB[PQ], qC[PQ];
+    auto issue = [&](int slot_i, float4& A, float4& B, float4& C) {
+        const int row = O + slot_i - 5 * k;
+        const bool rowA = row >= aLo && row < aHi, rowB = row >= bLo && row < bHi;
+        A = gld4(pA + ((aLane && rowA) ? row * ld + gx : 0));
+        const int ib = (useB && bLane && rowB) ? row * ld + gx : 0;
+        if constexpr ((FLAGS & F_GRAD) != 0) {
+            B = k > 0 ? gld4_nt(pB + ib) : gld4(pB + ib);
+            const bool cok = k == 0 && lcore && row >= Y0 && row < Y1;
+            C = gld4(d.grad + (cok ? row * ld + gx : 0));
+        } else {
+            B = gld4(pB + ib);
+        }
+    };
+
+    // ---- per-level constants
+    float srcadd = 0.f;
+    int srow = -1, sg4 = -1, sj = 0;
+    if constexpr ((FLAGS & F_SRC) != 0) {
+        if (k > 0 && active) {
+            srow = (int)(d.isrc / ld);
+            const int sx = (int)(d.isrc - (long long)srow * ld);
+            sg4 = sx >> 2;
+            sj = sx & 3;
+            srcadd = inject_term(gld1(d.w + t), gld1(d.vel + d.isrc), d.dt2);
+        }
+    }
+    float4 inj = make_float4(0.f, 0.f, 0.f, 0.f);
+    unsigned injm = 0;
+    if constexpr ((FLAGS & F_RESID) != 0) {
+        if (k > 0 && active && d.rrow >= Ok && d.rrow < Ek) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int x = gx + j;
+                if (lin && x >= d.rx0 && x < d.rx0 + d.nrec) {
+                    injm |= 1u << j;
+                    f4set(inj, j, inject_term(gld1(d.resid + (long long)t * d.nrec + (x - d.rx0)),
+                                              gld1(d.vel + (long long)d.rrow * ld + x), d.dt2));
+                }
+            }
+        }
+    }
+    int ckA = -1, ckB = -1;
+    if ((FLAGS & F_REC) != 0 && a.K > 0 && a.dir > 0 && k > 0) {
+        if ((t + 2) % a.K == 0 && (t + 2) / a.K < a.nseg) ckA = 2 * ((t + 2) / a.K);
+        if ((t + 1) % a.K == 0 && (t + 1) / a.K < a.nseg) ckB = 2 * ((t + 1) / a.K) + 1;
+    }
+
+    // LDS ring bases of this wave: it reads ring k-1 and writes ring k
+    const int curR = LY::CUR + (k > 0 ? k - 1 : 0) * 2 * PLANES + lane;
+    const int curW = LY::CUR + k * 2 * PLANES + lane;
+    const int prvR = LY::PRV + (k > 0 ? k - 1 : 0) * NPRV * PLANES + lane;
+    const int prvW = LY::PRV + k * NPRV * PLANES + lane;
+    const int grdR = LY::GRD + (k > 0 ? k - 1 : 0) * NGRD * PLANES + lane;
+    const int grdW = LY::GRD + k * NGRD * PLANES + lane;
+    const bool writes_next = k < T && k < a.nsteps;
+    const int ksb = LY::KS + (k > 0 ? k - 1 : 0) * 32;   // this level's Keys scratch (float4 index)
+
+    float4 win[PW];
+#pragma unroll
+    for (int q = 0; q < PW; ++q) win[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int q = 0; q < PD; ++q) issue(q, qA[q], qB[q], qC[q]);
+
+    const int ET = ey1 ? dimy : E - 4 * T;
+    const int nit = ET - O + 5 * T;
+    int c7 = 0, c6 = 0;   // iteration mod NPRV, mod NGRD
+
+    auto body = [&](int i, auto UC) {
+        constexpr int U = decltype(UC)::value;
+        issue(i + PD, qA[(U + PD) % PQ], qB[(U + PD) % PQ], qC[(U + PD) % PQ]);
+        if (k == 0) {
+            // loader: rows O + i of both input fields (and the gradient)
+            S[curW + (U % 2) * PLANES] = qB[U % PQ];
+            S[prvW + c7 * PLANES] = qA[U % PQ];
+            if constexpr ((FLAGS & F_GRAD) != 0) S[grdW + c6 * PLANES] = qC[U % PQ];
+        } else {
+            // append row r+4 of level k-1 (written by wave k-1 last iteration)
+            win[U] = S[curR + ((U + 1) % 2) * PLANES];
+            const int r = O + i - 5 * k;
+            if (active && r >= Ok && r < Ek) {
+                const float4 C = win[(U + PW - 4) % PW];
+                const int c7r = c7 + 2 >= NPRV ? c7 + 2 - NPRV : c7 + 2;   // (i-5) mod 7
+                const float4 P = S[prvR + c7r * PLANES];
+                const float4 M = qA[U % PQ];
+                float4 un;
+                if (r < NE) {
+                    un = make_float4(0.f, 0.f, 0.f, 0.f);
+                } else if (r >= dimy - NE) {
+                    // bottom Keys rows: every cell, corners included
+                    const int c7r1 = c7 + 1 >= NPRV ? c7 + 1 - NPRV : c7 + 1;   // (i-6) mod 7
+                    const float4 P1 = S[prvR + c7r1 * PLANES];
+                    const float4 C1 = win[(U + PW - 5) % PW], C2 = win[(U + PW - 6) % PW];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        f4set(un, j, gx + j < dimx ? keys_update(f4at(P, j), f4at(C, j), f4at(P1, j), f4at(C1, j),
+                                                                 f4at(C2, j), f4at(M, j), d.h2, d.dt2, d.hdt)
+                                                   : 0.f);
+                } else {
+                    float4 up[NE], dn[NE];
+#pragma unroll
+                    for (int q = 1; q <= NE; ++q) {
+                        up[q - 1] = win[(U + PW - 4 - q) % PW];
+                        dn[q - 1] = win[(U + PW - 4 + q) % PW];
+                    }
+                    const float4 Lf = left4(C), Rf = right4(C);
+                    un = lap_update4<NE>(up, dn, Lf, C, Rf, P, M);
+                    // Left / right Keys columns (x < NE, x >= dimx-NE) and the
+                    // row padding (x >= dimx).  The Keys cells sit in at most
+                    // three lanes (lane 0, and the one or two lanes holding
+                    // x >= dimx-NE); computing them there would take four
+                    // sequential updates per lane.  Instead those owner lanes
+                    // put their row segment (x-4 .. x+7 of u_prev and u_cur, and
+                    // the model) in a small LDS scratch of this wave, one
+                    // worker lane per Keys cell computes it (lanes 0..NE-1 the
+                    // left cells, lanes 4..3+NE the right ones), and the owners
+                    // read the results back.  LDS operations of one wave are
+                    // performed in order, so no barrier is needed.
+                    if (keys_left || keys_right) {
+                        const float4 PL = left4(P), PR = right4(P);
+                        float* const ks = (float*)&S[ksb];
+                        const bool ownL = keys_left && lane == 0;
+                        const bool ownR = keys_right && g4 >= kx4 && g4 < dimx4;
+                        const int sR = 1 + (g4 - kx4);
+                        if (ownL) {
+                            float4* o = (float4*)ks;
+                            o[0] = PL; o[1] = P; o[2] = PR; o[3] = Lf; o[4] = C; o[5] = Rf; o[6] = M;
+                        }
+                        if (ownR) {
+                            float4* o = (float4*)(ks + 32 * sR);
+                            o[0] = PL; o[1] = P; o[2] = PR; o[3] = Lf; o[4] = C; o[5] = Rf; o[6] = M;
+                        }
+                        int wx = -1, ws = 0, wd = 0;
+                        if (keys_left && lane < NE) {
+                            wx = lane;
+                            wd = 1;
+                        }
+                        if (keys_right && lane >= 4 && lane < 4 + NE) {
+                            wx = dimx - NE + (lane - 4);
+                            ws = 1 + ((wx >> 2) - kx4);
+                            wd = -1;
+                        }
+                        if (wx >= 0) {
+                            const float* o = ks + 32 * ws;
+                            const int c = wx & 3;
+                            // o[0..11]: u_prev at x-4..x+7; o[12..23]: u_cur; o[24..27]: model
+                            const float kv = keys_update(o[4 + c], o[16 + c], o[4 + c + wd], o[16 + c + wd],
+                                                         o[16 + c + 2 * wd], o[24 + c], d.h2, d.dt2, d.hdt);
+                            ks[96 + 4 * ws + c] = kv;
+                        }
+                        if (ownL) {
+                            const float4 kr = *(const float4*)(ks + 96);
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                if (j < NE) f4set(un, j, f4at(kr, j));
+                        }
+                        if (ownR) {
+                            const float4 kr = *(const float4*)(ks + 96 + 4 * sR);
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                const int x = gx + j;
+                                f4set(un, j, x >= dimx ? 0.f : (x >= dimx - NE ? f4at(kr, j) : f4at(un, j)));
+                            }
+                        }
+                    }
+                }
+                if constexpr ((FLAGS & F_SRC) != 0) {
+                    if (r == srow && g4 == sg4) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (j == sj) f4set(un, j, f4at(un, j) + srcadd);
+                    }
+                }
+                if constexpr ((FLAGS & F_RESID) != 0) {
+                    if (r == d.rrow) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (injm & (1u << j)) f4set(un, j, f4at(un, j) + f4at(inj, j));
+                    }
+                }
+                const bool orow = r >= Y0 && r < Y1;
+                const int i0 = r * ld + gx;
+                if (orow && lcore) {
+                    if constexpr ((FLAGS & F_REC) != 0) {
+                        if (r == d.rrow) {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                const int x = gx + j;
+                                if (x >= d.rx0 && x < d.rx0 + d.nrec)
+                                    gst1(d.frd + (long long)t * d.nrec + (x - d.rx0), f4at(un, j));
+                            }
+                        }
+                    }
+                    if constexpr ((FLAGS & F_VIRT) != 0) {
+                        const float4 c3 = qB[U % PQ];
+                        float4 vs;
+                        vs.x = c3.x * ((P.x - 2.f * C.x) + un.x);
+                        vs.y = c3.y * ((P.y - 2.f * C.y) + un.y);
+                        vs.z = c3.z * ((P.z - 2.f * C.z) + un.z);
+                        vs.w = c3.w * ((P.w - 2.f * C.w) + un.w);
+                        gst4_nt(d.virt + voff + i0, vs);
+                    }
+                    if (ckA >= 0) gst4_nt(d.ckpt + ckA * d.fstride + i0, un);
+                    if (ckB >= 0) gst4_nt(d.ckpt + ckB * d.fstride + i0, un);
+                    if (k == a.nsteps) {
+                        gst4(out_prev + i0, C);
+                        gst4(out_cur + i0, un);
+                    }
+                }
+                if constexpr ((FLAGS & F_GRAD) != 0) {
+                    const int c6r = c6 + 1 >= NGRD ? c6 + 1 - NGRD : c6 + 1;   // (i-5) mod 6
+                    const float4 gin = S[grdR + c6r * PLANES];
+                    const float4 v = qB[U % PQ];
+                    float4 g;
+                    g.x = gin.x + (-v.x) * un.x;
+                    g.y = gin.y + (-v.y) * un.y;
+                    g.z = gin.z + (-v.z) * un.z;
+                    g.w = gin.w + (-v.w) * un.w;
+                    if (k == a.nsteps) {
+                        if (orow && lcore) gst4(d.grad + i0, g);
+                    } else {
+                        S[grdW + c6 * PLANES] = g;
+                    }
+                }
+                if (writes_next) {
+                    S[curW + (U % 2) * PLANES] = un;
+                    S[prvW + c7 * PLANES] = C;
+                }
+            }
+        }
+        c7 = c7 + 1 == NPRV ? 0 : c7 + 1;
+        c6 = c6 + 1 == NGRD ? 0 : c6 + 1;
+        __syncthreads();
+    };
+
+    for (int base = 0; base < nit; base += PW) {
+        body(base + 0, IC<0>{});
+        body(base + 1, IC<1>{});
+        body(base + 2, IC<2>{});
+        body(base + 3, IC<3>{});
+        body(base + 4, IC<4>{});
+        body(base + 5, IC<5>{});
+        body(base + 6, IC<6>{});
+        body(base + 7, IC<7>{});
+        body(base + 8, IC<8>{});
+        body(base + 9, IC<9>{});
+    }
+}
+
+}  // namespace tw

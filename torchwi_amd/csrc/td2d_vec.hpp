@@ -227,19 +227,24 @@ __global__ __launch_bounds__(VTHR) void k_vstep(const ShotDesc* __restrict__ des
 }
 
 // Derived model arrays of one padded model: m = (v*v)*dtoh2 and
-// c3 = cons/((v*v)*v), exactly the float products the step forms.
-// Columns at or beyond dimx (row padding) get zeros.
+// c3 = cons/((v*v)*v), exactly the float products the step forms.  Keys
+// boundary cells (the bottom ne rows, and the left/right ne columns below the
+// top ne rows), which never use m, carry v itself in m, so that a kernel
+// reads one model value per cell whatever its region.  Columns at or beyond
+// dimx (row padding) get zeros.
 __global__ void k_model_prep(const float* __restrict__ v, float* __restrict__ m,
-                             float* __restrict__ c3, int dimx, int dimy, int ld, float dtoh2,
-                             float cons)
+                             float* __restrict__ c3, int dimx, int dimy, int ld, int ne,
+                             float dtoh2, float cons)
 {
     const long long n = (long long)dimy * ld;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         const int x = (int)(i % ld);
+        const int y = (int)(i / ld);
         const float vv = v[i];
         const bool in = x < dimx;
-        m[i] = in ? (vv * vv) * dtoh2 : 0.f;
+        const bool keys = y >= dimy - ne || (y >= ne && (x < ne || x >= dimx - ne));
+        m[i] = in ? (keys ? vv : (vv * vv) * dtoh2) : 0.f;
         c3[i] = in ? cons / ((vv * vv) * vv) : 0.f;
     }
 }
