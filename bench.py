@@ -49,6 +49,8 @@ def parse():
                     help="time steps per launch (2..4: temporally blocked kernel)")
     ap.add_argument("--tile-rows", type=int, default=0,
                     help="tile height (one step per launch: 32 or 64; blocked kernel: rows per chunk); 0 automatic")
+    ap.add_argument("--tile-cols", type=int, default=0,
+                    help="one-step kernel tile width: 64 or 128; 0 automatic")
     ap.add_argument("--graph-steps", type=int, default=100,
                     help="time steps per captured hipGraph block (0: one direct launch per step)")
     ap.add_argument("--compare", type=int, default=1,
@@ -111,14 +113,15 @@ def cpu_baseline():
                       "forward with full storage + adjoint gradient; median of 5 runs (%.3f s each)" % t}
 
 
-def run_batched(c, K, steps, warmup, timing, graph_steps, dev, time_block=1, tile_rows=0):
+def run_batched(c, K, steps, warmup, timing, graph_steps, dev, time_block=1, tile_rows=0, tile_cols=0):
     """Time `steps` misfit gradients of the batched engine; returns stats."""
     import torch
     from torchwi_amd import Time2dBatch
     order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
     S = len(c["sx"])
     model = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K,
-                        graph_steps=graph_steps, time_block=time_block, tile_rows=tile_rows)
+                        graph_steps=graph_steps, time_block=time_block, tile_rows=tile_rows,
+                        tile_cols=tile_cols)
     vel = torch.from_numpy(c["vel"]).to(dev)
     rng = np.random.default_rng(7)
     vtrue = torch.from_numpy(c["vel"] * (1.0 + 0.02 * rng.standard_normal(c["vel"].shape)).astype(np.float32)).to(dev)
@@ -182,13 +185,14 @@ def run_plain(c, nshots, dev):
             "s_per_shot": round(el / nshots, 4)}
 
 
-def load_traffic(kernel_key):
+def load_traffic(mode_key, kernel_key):
     """HBM bytes per launch of a step kernel at this workload, from the
-    calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs kept in profiles/."""
+    calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs kept in profiles/
+    (mode_key: "full_storage" or "checkpointed")."""
     path = os.path.join(ROOT, "profiles", "traffic_k_vstep_384x1024_32shots.json")
     try:
         with open(path) as f:
-            return json.load(f)["kernels"].get(kernel_key)
+            return json.load(f)[mode_key].get(kernel_key)
     except (OSError, ValueError, KeyError):
         return None
 
@@ -211,7 +215,8 @@ def main():
     tb = args.time_block
     kck = args.ckpt_every or default_ckpt_interval(nt, tb)
     K = {"full": 0, "ckpt": kck, "auto": "auto"}[args.mode]
-    r = run_batched(c, K, args.steps, args.warmup, not args.no_timing, args.graph_steps, dev, tb, args.tile_rows)
+    r = run_batched(c, K, args.steps, args.warmup, not args.no_timing, args.graph_steps, dev, tb, args.tile_rows,
+                    args.tile_cols)
     S, kstate, value, elapsed, lat = r["S"], r["K"], r["value"], r["elapsed"], r["lat"]
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
@@ -224,7 +229,8 @@ def main():
                                % (S, ny, nx, r["dimy"], r["dimx"], h, dt, nt, order,
                                   "full virtual-source storage" if kstate == 0 else "checkpoint every %d steps + recompute" % kstate),
                    "shots": S, "nz": ny, "nx": nx, "nt": nt, "order": order, "ckpt_every": kstate,
-                   "graph_steps": args.graph_steps, "time_block": tb, "tile_rows": args.tile_rows},
+                   "graph_steps": args.graph_steps, "time_block": tb, "tile_rows": args.tile_rows,
+                   "tile_cols": args.tile_cols},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "gradient_shots_per_s": round(r["shots_per_s"], 3),
         "effective_hbm_frac": round(value * 1e9 * (BYTES_GRAD_STEP / 2.0) / (HBM_PEAK_GBS * 1e9), 4),
@@ -243,14 +249,15 @@ def main():
         dom = max(per, key=lambda k: per[k]["avg_us"] * per[k]["launches"])
         b = {"fwd": BYTES_FWD, "fwd_virt": BYTES_FWD_VIRT, "recomp": BYTES_FWD_VIRT, "adj": BYTES_ADJ}[dom]
         ach = per[dom]["achieved_GBs"]
-        tr = load_traffic(dom if tb <= 1 else "%s_tb%d" % (dom, tb)) if (nt, S, order) == (4000, 32, 8) else None
-        out["roofline"] = {"bound": "hbm", "kernel": ("k_tstep<4,%s,%d>" % (dom, tb)) if tb > 1 else "k_vstep<4,%s>" % dom,
+        tr = (load_traffic("full_storage" if kstate == 0 else "checkpointed", dom)
+              if (nt, S, order, tb, args.tile_rows, args.tile_cols) == (4000, 32, 8, 1, 0, 0) else None)
+        out["roofline"] = {"bound": "hbm", "kernel": ("k_pipe<4,%s,%d>" % (dom, tb)) if tb > 1 else "k_vstep<4,%s>" % dom,
                            "achieved": ach,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                            "traffic": tr, "bytes_per_cell": b, "cells_per_launch": cells_launch}
     if args.compare:
         if kstate == 0:
-            rc = run_batched(c, kck, 1, 0, False, args.graph_steps, dev, tb, args.tile_rows)
+            rc = run_batched(c, kck, 1, 0, False, args.graph_steps, dev, tb, args.tile_rows, args.tile_cols)
             out["checkpointed"] = {"ckpt_every": rc["K"], "value": round(rc["value"], 3),
                                    "ms_per_step": round(rc["elapsed"] * 1e3, 3),
                                    "gradient_shots_per_s": round(rc["shots_per_s"], 3),

@@ -108,7 +108,9 @@ typedef struct TwPlanSpec {
     float h, dt;      /* grid spacing, time step */
     int ckpt_every;   /* K > 0: checkpoint every K steps and recompute;
                          0: full virtual-source storage (nt slots) */
-    int tile_rows;    /* step-kernel tile height: 32 or 64, 0 = automatic */
+    int tile_rows;    /* one-step kernel tile height (64 x 32, 64 x 64, 128 x 16 or
+                         128 x 32 with tile_cols), or the blocked kernel's
+                         row-chunk height (>= 16); 0 = automatic */
     int graph_steps;  /* > 0: time steps are launched as replays of hipGraphs
                          captured once per plan (this many steps per forward /
                          full-storage adjoint block; one graph per checkpoint
@@ -119,6 +121,7 @@ typedef struct TwPlanSpec {
     int forward_only; /* 1: modelling only -- no virtual-source storage, no
                          adjoint ring (virt, lam, ckpt may be NULL; ckpt_every
                          must be 0; tw_plan_backward is refused) */
+    int tile_cols;    /* one-step kernel tile width: 64 or 128, 0 = automatic */
 } TwPlanSpec;
 
 typedef struct TwShotSpec {

@@ -225,20 +225,23 @@ def test_invalid_arguments_raise():
                           8, 20, 20, 10, 10.0, 1e-3, 50.0, 20.0, 20.0)
 
 
-@pytest.mark.parametrize("order,nx,ny,tile_rows", [
-    (8, 150, 70, 32), (8, 150, 70, 64), (4, 131, 57, 32), (2, 67, 29, 64), (8, 61, 32, 0)])
-def test_batch_kernel_shapes_bitexact(order, nx, ny, tile_rows):
+@pytest.mark.parametrize("order,nx,ny,tile_rows,tile_cols", [
+    (8, 150, 70, 32, 0), (8, 150, 70, 64, 0), (4, 131, 57, 32, 0), (2, 67, 29, 64, 0), (8, 61, 32, 0, 0),
+    (8, 301, 70, 32, 128), (8, 150, 45, 16, 128), (4, 131, 57, 0, 128), (2, 67, 29, 16, 128),
+    (8, 150, 70, 16, 64), (4, 77, 41, 16, 0), (8, 61, 32, 8, 128)])
+def test_batch_kernel_shapes_bitexact(order, nx, ny, tile_rows, tile_cols):
     """The vectorised batched kernel over odd widths (dimx % 4 != 0), several
-    tiles in x and y, both tile heights: gathers and per-shot gradients equal
-    the oracle bit for bit."""
+    tiles in x and y, every tile shape (64 x 16/32/64, 128 x 8/16/32):
+    gathers and per-shot gradients equal the oracle bit for bit."""
     from torchwi_amd import Time2dBatch
     c = small_case(order, nx=nx, ny=ny, nt=140, seed=nx + ny)
     sx = [(i + 0.5) * c["h"] for i in (1, nx // 2, nx - 2)]
     sy = [(i + 0.5) * c["h"] for i in (0, 5, ny - 1)]
     ry = [(i + 0.5) * c["h"] for i in (0, 3, ny - 1)]
-    print(f"case order={order} nx={nx} ny={ny} tile_rows={tile_rows}", flush=True)
+    print(f"case order={order} nx={nx} ny={ny} tile_rows={tile_rows} tile_cols={tile_cols}", flush=True)
     w = torch.from_numpy(c["w"])
-    m = Time2dBatch(nx, ny, c["h"], w, c["dt"], order, DEV, ckpt_every=0, tile_rows=tile_rows)
+    m = Time2dBatch(nx, ny, c["h"], w, c["dt"], order, DEV, ckpt_every=0, tile_rows=tile_rows,
+                    tile_cols=tile_cols)
     vel = torch.from_numpy(c["vel"]).to(DEV)
     frd = m.modeling(vel, sx, sy, ry).clone()
     d_obs = torch.zeros_like(frd)

@@ -1,8 +1,7 @@
 """Pick the step-kernel configuration of a shot plan by timing candidates.
 
-The candidates are the kernel's tile height for one step per launch
-(32 or 64 rows, replayed as hipGraph blocks) and the temporally blocked
-kernel (2..4 steps per launch).  Each candidate runs one misfit gradient
+The candidates are tile shapes of the one-step kernel (replayed as hipGraph
+blocks) and the temporally blocked kernel (several steps per launch).  Each candidate runs one misfit gradient
 (forward + adjoint) on a proxy plan with the same shots, grids and order but
 a short time axis, so the choice reflects the real launch shapes at a small
 fraction of the cost.  All candidates compute bit-identical results; only
@@ -15,9 +14,10 @@ import torch
 from .shots import ShotPlan, default_ckpt_interval
 
 CANDIDATES = (
-    dict(tile_rows=64, time_block=1),
-    dict(tile_rows=32, time_block=1),
-    dict(tile_rows=0, time_block=3),
+    dict(tile_cols=64, tile_rows=16, time_block=1),
+    dict(tile_cols=64, tile_rows=32, time_block=1),
+    dict(tile_cols=128, tile_rows=8, time_block=1),
+    dict(tile_cols=0, tile_rows=0, time_block=3),
 )
 
 
@@ -31,7 +31,8 @@ def autotune(order, h, dt, models, shots, device, nt_probe=96, candidates=CANDID
     for cand in candidates:
         K = default_ckpt_interval(nt_probe, cand["time_block"])
         p = ShotPlan(order, nt_probe, h, dt, [(nx, ny) for nx, ny in models], probe_shots, K, device,
-                     tile_rows=cand["tile_rows"], time_block=cand["time_block"])
+                     tile_rows=cand["tile_rows"], time_block=cand["time_block"],
+                     tile_cols=cand["tile_cols"])
         for i in range(len(models)):
             p.models[i].vel.fill_(2000.0)
         p.forward()
@@ -43,7 +44,7 @@ def autotune(order, h, dt, models, shots, device, nt_probe=96, candidates=CANDID
             p.backward()
         torch.cuda.synchronize(device)
         dt_s = (time.perf_counter() - t0) / repeats
-        label = "tile_rows=%d,time_block=%d" % (cand["tile_rows"], cand["time_block"])
+        label = "tile=%dx%d,time_block=%d" % (cand["tile_cols"], cand["tile_rows"], cand["time_block"])
         times[label] = round(dt_s, 5)
         if best_t is None or dt_s < best_t:
             best, best_t = cand, dt_s

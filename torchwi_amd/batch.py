@@ -80,7 +80,8 @@ class Time2dBatch(torch.nn.Module):
     """
 
     def __init__(self, nx, ny, h, w, dt, order, device="cuda", ckpt_every="auto",
-                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK):
+                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK,
+                 tile_cols=0):
         super().__init__()
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -96,6 +97,7 @@ class Time2dBatch(torch.nn.Module):
         self.ckpt_every = ckpt_every
         self.hbm_fraction = hbm_fraction
         self.tile_rows = tile_rows
+        self.tile_cols = tile_cols
         self._plan = None
         self._key = None
 
@@ -117,7 +119,7 @@ class Time2dBatch(torch.nn.Module):
             torch.cuda.empty_cache()
             self._plan = ShotPlan(self.order, self.nt, self.h, self.dt, [(self.nx, self.ny)], shots, K,
                                   self.device, tile_rows=self.tile_rows, graph_steps=self.graph_steps,
-                                  time_block=self.time_block)
+                                  time_block=self.time_block, tile_cols=self.tile_cols)
             self._key = key
         self._plan.models[0].load(vel)
         return self._plan
@@ -172,7 +174,8 @@ class Time2dWindows(torch.nn.Module):
     """
 
     def __init__(self, nx, ny, h, w, dt, order, windows, device="cuda", ckpt_every="auto",
-                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK):
+                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK,
+                 tile_cols=0):
         super().__init__()
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -191,7 +194,7 @@ class Time2dWindows(torch.nn.Module):
         K = _choose_ckpt(ckpt_every, plan_bytes([(a, b, order) for a, b in models], shots, self.nt, 0),
                          self.nt, self.device, hbm_fraction, time_block=time_block)
         self.plan = ShotPlan(order, self.nt, h, dt, models, shots, K, self.device, tile_rows=tile_rows,
-                             graph_steps=graph_steps, time_block=time_block)
+                             graph_steps=graph_steps, time_block=time_block, tile_cols=tile_cols)
 
     def load(self, vel):
         for i, q in enumerate(self.windows):

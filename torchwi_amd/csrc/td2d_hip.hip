@@ -624,7 +624,8 @@ struct Plan {
     Graph g_fwd, g_bwd, g_recomp, g_adj;
     int* d_ctl = nullptr;             // device control words (block / segment index)
     int order = 0, nt = 0, K = 0, nseg = 0, nshots = 0, ntiles = 0, per_xcd = 0;
-    int ryv = 4;                      // rows per thread of the step kernel (tile height 16*ryv)
+    int ryv = 4;                      // rows per thread of the step kernel
+    int cx4 = 16;                     // float4 columns per tile (tile 4*cx4 x (256/cx4)*ryv)
     long long max_field = 0;
     std::vector<ShotDesc> host;
     std::vector<int2> tiles;          // (shot, by << 16 | bx), dealing order
@@ -643,20 +644,16 @@ struct Plan {
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-// Tile height for a set of grids: the smaller total of staged rows
-// (tiles_y * (tile_rows + 2*halo)), ties to the taller tile.
-int choose_ryv(const std::vector<int>& dimys, int ne)
+// Tile shape of the one-step kernel: 64 x 16 (one row per thread).  The
+// kernel is bound by how many bytes are in flight, and the shortest tile
+// keeps the fewest registers per thread, so the most workgroups per CU; its
+// larger halo share (24 staged rows per 16) is read from L2.  Measured on
+// MI355X, 32 shots of 1032 x 392: 64 x 16 beats 64 x 32 and 64 x 64 by 6% and
+// 28% (forward with storage + adjoint), and the 128-wide tiles as well.
+void choose_tile(int& cx4, int& ryv)
 {
-    const int halo = ne < 2 ? 2 : ne;
-    long long best = -1;
-    int pick = 4;
-    for (int ryv : {4, 2}) {
-        const int ty = 16 * ryv;
-        long long cost = 0;
-        for (int dy : dimys) cost += (long long)((dy + ty - 1) / ty) * (ty + 2 * halo);
-        if (best < 0 || cost < best) { best = cost; pick = ryv; }
-    }
-    return pick;
+    cx4 = 16;
+    ryv = 1;
 }
 
 // Resident workgroups of the temporally blocked kernel on the device (the
@@ -780,7 +777,6 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
     p.host.resize(nshots);
     p.tiles.clear();
     p.models.clear();
-    std::vector<int> dimys;
     auto al16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
     for (int s = 0; s < nshots; ++s) {
         const TwShotSpec& q = shots[s];
@@ -788,7 +784,7 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
         if (q.ld < q.dimx || q.ld % 4 != 0)
             return fail(TW_EINVAL, "shot %d: row stride ld=%d must be >= dimx=%d and a multiple of 4",
                         s, q.ld, q.dimx);
-        if (q.dimx >= (1 << 16) * VTX / 2 || q.dimy >= (1 << 15) * 16)
+        if (q.dimx >= (1 << 16) * 32 || q.dimy >= (1 << 15) * 16)
             return fail(TW_EINVAL, "shot %d: grid %dx%d too large for the tile table", s, q.dimx, q.dimy);
         Cells c = truncate_positions(q.sx, q.sy, q.ry, spec->h);
         if ((rc = check_src(c, spec->order, q.dimx, q.dimy))) return rc;
@@ -842,21 +838,32 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
         }
         if (!seen) {
             p.models.push_back({q.vel, q.vm, q.vc3, q.dimx, q.dimy, q.ld, d.dtoh2, d.cons});
-            dimys.push_back(q.dimy);
         }
     }
     if (p.tb > 1) return pipe_tiles(p, spec->tile_rows);
-    if (spec->tile_rows == 32 || spec->tile_rows == 64) p.ryv = spec->tile_rows / 16;
-    else if (spec->tile_rows == 0) p.ryv = choose_ryv(dimys, ne);
-    else return fail(TW_EINVAL, "tile_rows must be 0, 32 or 64 (got %d)", spec->tile_rows);
-    const int ty = 16 * p.ryv;
+    if (spec->tile_cols != 0 && spec->tile_cols != 64 && spec->tile_cols != 128)
+        return fail(TW_EINVAL, "tile_cols must be 0, 64 or 128 (got %d)", spec->tile_cols);
+    if (spec->tile_rows != 0 && spec->tile_rows != 8 && spec->tile_rows != 16 && spec->tile_rows != 32 &&
+        spec->tile_rows != 64)
+        return fail(TW_EINVAL, "tile_rows must be 0, 8, 16, 32 or 64 (got %d)", spec->tile_rows);
+    choose_tile(p.cx4, p.ryv);
+    if (spec->tile_cols) p.cx4 = spec->tile_cols / 4;
+    if (spec->tile_rows) {
+        p.ryv = spec->tile_rows / (VTHR / p.cx4);
+        if (p.ryv != 1 && p.ryv != 2 && p.ryv != 4)
+            return fail(TW_EINVAL, "tile %dx%d is not a one-step tile (64 x 16/32/64 or 128 x 8/16/32)",
+                        4 * p.cx4, spec->tile_rows);
+    } else if (spec->tile_cols) {
+        p.ryv = 4;
+    }
+    const int tw = 4 * p.cx4, ty = (VTHR / p.cx4) * p.ryv;
     // Deal tiles model by model; within a model spatial-tile-major and
     // shot-minor, so consecutive workgroups of an XCD share model rows.
     for (const ModelRef& m : p.models) {
         std::vector<int> group;
         for (int s = 0; s < nshots; ++s)
             if (p.host[s].vel == m.vel) group.push_back(s);
-        const int ntx = (m.dimx + VTX - 1) / VTX, nty = (m.dimy + ty - 1) / ty;
+        const int ntx = (m.dimx + tw - 1) / tw, nty = (m.dimy + ty - 1) / ty;
         for (int by = 0; by < nty; ++by)
             for (int bx = 0; bx < ntx; ++bx)
                 for (int s : group) p.tiles.push_back(make_int2(s, (by << 16) | bx));
@@ -871,15 +878,20 @@ size_t plan_bytes(const Plan& p)
     return align256(sizeof(ShotDesc) * p.nshots) + align256(sizeof(int2) * p.ntiles) + 256;
 }
 
+template <int NE, unsigned FLAGS, int CX4>
+void launch_batch_nfc(const Plan& p, const StepArgs& a, hipStream_t s)
+{
+    const dim3 g(p.per_xcd * NXCD), b(VTHR);
+    if (p.ryv == 4) hipLaunchKernelGGL((k_vstep<NE, FLAGS, 4, CX4>), g, b, 0, s, p.d_desc, p.d_tiles, a);
+    else if (p.ryv == 2) hipLaunchKernelGGL((k_vstep<NE, FLAGS, 2, CX4>), g, b, 0, s, p.d_desc, p.d_tiles, a);
+    else hipLaunchKernelGGL((k_vstep<NE, FLAGS, 1, CX4>), g, b, 0, s, p.d_desc, p.d_tiles, a);
+}
+
 template <int NE, unsigned FLAGS>
 void launch_batch_nf(const Plan& p, const StepArgs& a, hipStream_t s)
 {
-    if (p.ryv == 4)
-        hipLaunchKernelGGL((k_vstep<NE, FLAGS, 4>), dim3(p.per_xcd * NXCD), dim3(VTHR), 0, s,
-                           p.d_desc, p.d_tiles, a);
-    else
-        hipLaunchKernelGGL((k_vstep<NE, FLAGS, 2>), dim3(p.per_xcd * NXCD), dim3(VTHR), 0, s,
-                           p.d_desc, p.d_tiles, a);
+    if (p.cx4 == 32) launch_batch_nfc<NE, FLAGS, 32>(p, a, s);
+    else launch_batch_nfc<NE, FLAGS, 16>(p, a, s);
 }
 
 template <int NE>

@@ -2,8 +2,9 @@
 //
 // Same per-cell arithmetic as step_tile (bit-identical results), organised
 // for HBM throughput:
-//   * a workgroup (256 threads) owns a 64 x (16*RYV) tile; each thread owns
-//     4 consecutive columns (one float4) of RYV consecutive rows;
+//   * a workgroup (256 threads) owns a (4*CX4) x (256/CX4*RYV) tile (CX4 =
+//     16 or 32 float4 columns); each thread owns 4 consecutive columns (one
+//     float4) of RYV consecutive rows;
 //   * every global load the thread needs (u_prev, model, virt, grad) is
 //     issued first, for all its rows, before the current wavefield is staged
 //     in LDS, so that many 16-byte loads per lane are in flight;
@@ -18,8 +19,7 @@
 
 namespace tw {
 
-constexpr int VTX = 64;     // tile width in cells (16 threads x float4)
-constexpr int VTHR = 256;
+constexpr int VTHR = 256;   // threads of the one-step batched kernel
 
 // Loads/stores through the global address space: the pointers come from a
 // descriptor in memory, so the compiler cannot infer it and would emit flat
@@ -74,13 +74,14 @@ __device__ __forceinline__ void f4set(float4& v, int j, float x)
     else v.w = x;
 }
 
-template <int NE, unsigned FLAGS, int RYV>
+template <int NE, unsigned FLAGS, int RYV, int CX4>
 __global__ __launch_bounds__(VTHR) void k_vstep(const ShotDesc* __restrict__ descs,
                                                 const int2* __restrict__ tiles, StepArgs a)
 {
     constexpr int HALO = NE < 2 ? 2 : NE;   // Keys reads two cells inward
-    constexpr int TYV = 16 * RYV;
-    constexpr int W4 = VTX / 4 + 2;         // staged row: 4 halo columns each side
+    constexpr int TR = VTHR / CX4;          // thread rows
+    constexpr int TYV = TR * RYV;
+    constexpr int W4 = CX4 + 2;             // staged row: 4 halo columns each side
     constexpr int W = 4 * W4;               // in floats
     constexpr int H = TYV + 2 * HALO;
     __shared__ float4 T4[H * W4];
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(VTHR) void k_vstep(const ShotDesc* __restrict__ des
     if (L >= a.ntiles || !resolve_step(a)) return;
     const int2 e = tiles[L];
     const ShotDesc& d = descs[e.x];
-    const int x0 = (e.y & 0xffff) * VTX, y0 = (e.y >> 16) * TYV;
+    const int x0 = (e.y & 0xffff) * 4 * CX4, y0 = (e.y >> 16) * TYV;
     const int dimx = d.dimx, dimy = d.dimy;
     const long long ld = d.ld;
 
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(VTHR) void k_vstep(const ShotDesc* __restrict__ des
     const float* __restrict__ uo = a.adj ? pick3(d.lam, k1) : pick3(d.u, k1);
     float* __restrict__ up = a.adj ? pick3(d.lam, k2) : pick3(d.u, k2);
 
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int tx = threadIdx.x % CX4, ty = threadIdx.x / CX4;
     const int gx = x0 + 4 * tx;
     const int gy0 = y0 + ty * RYV;
     const bool colok = gx < dimx;

@@ -104,10 +104,13 @@ class ShotPlan:
     time_block: time steps advanced per launch by the temporally blocked
             kernel (2..4; 1: one step per launch).  ckpt_every must then be
             a multiple of it.
+    tile_rows, tile_cols: tile shape of the one-step kernel (64x32, 64x64,
+            128x16, 128x32; 0 automatic) or, with time_block > 1, the row-chunk
+            height of the blocked kernel.
     """
 
     def __init__(self, order, nt, h, dt, models, shots, ckpt_every, device, tile_rows=0,
-                 graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK, forward_only=False):
+                 graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK, forward_only=False, tile_cols=0):
         self.order, self.nt, self.h, self.dt = order, int(nt), float(h), float(dt)
         self.device = torch.device(device)
         self.K = int(ckpt_every)
@@ -147,7 +150,8 @@ class ShotPlan:
         self._wavelets = []
         spec = _lib.TwPlanSpec(order=order, nt=self.nt, h=self.h, dt=self.dt, ckpt_every=self.K,
                                tile_rows=int(tile_rows), graph_steps=int(graph_steps),
-                               time_block=int(time_block), forward_only=int(self.forward_only))
+                               time_block=int(time_block), forward_only=int(self.forward_only),
+                               tile_cols=int(tile_cols))
         self.graph_steps = int(graph_steps)
         self.time_block = int(time_block)
         cs = (_lib.TwShotSpec * S)()

@@ -34,7 +34,7 @@ def main(argv=None):
     out = {"workload": "%d shots on windows of a %dx%d model, h=%g m, dt=%g s, nt=%d, order %d, "
                        "receiver windows %d..%d cells" % (len(wins), ny, nx, h, dt, nt, order,
                                                           min(q["nrec"] for q in wins), max(q["nrec"] for q in wins))}
-    cfg = dict(tile_rows=0, time_block=1)
+    cfg = dict(tile_cols=0, tile_rows=0, time_block=1)
     if not args.no_autotune:
         models = [(q["x1"] - q["x0"], ny) for q in wins]
         shots = [dict(model=i, sx=q["sx"], sy=q["sy"], ry=q["ry"], rx0=q["rx0"], nrec=q["nrec"])
