@@ -19,7 +19,7 @@ gradients in shot order.
 """
 import torch
 
-from .shots import GRAPH_STEPS, TIME_BLOCK, ShotPlan, default_ckpt_interval, plan_bytes
+from .shots import GRAPH_STEPS, ShotPlan, auto_time_block, default_ckpt_interval, plan_bytes
 
 __all__ = ["Time2dBatch", "Time2dWindows", "default_ckpt_interval"]
 
@@ -37,6 +37,18 @@ def _wavelet(w):
     if w.dtype != torch.float32:
         raise RuntimeError("w: expected scalar type Float but found %s" % w.dtype)
     return w.contiguous()
+
+
+def _resolve_time_block(time_block, cells_per_step, ckpt_every):
+    """Steps per launch: the given number, or for "auto" the blocked kernel
+    on large plans (shots.auto_time_block) unless an explicit checkpoint
+    interval is not a multiple of its block."""
+    if time_block != "auto":
+        return int(time_block)
+    tb = auto_time_block(cells_per_step)
+    if not isinstance(ckpt_every, str) and int(ckpt_every) > 0 and int(ckpt_every) % tb:
+        return 1
+    return tb
 
 
 def _choose_ckpt(ckpt_every, need_full, nt, device, hbm_fraction, held=0, time_block=1):
@@ -75,12 +87,15 @@ class Time2dBatch(torch.nn.Module):
     ``ckpt_every``: 0 = full virtual-source storage, K > 0 = checkpoint
     every K steps and recompute, "auto" = full storage when it fits in
     ``hbm_fraction`` of free device memory, else K = ceil(sqrt(nt)).
+    ``time_block``: time steps per kernel launch; "auto" picks the
+    temporally blocked kernel (3 steps) for plans of at least
+    shots.AUTO_BLOCK_CELLS cells per step and one step per launch below.
     The returned gathers are the plan's buffer: the next forward overwrites
     them (as the reference's module buffer, TimeFunction.py:128).
     """
 
     def __init__(self, nx, ny, h, w, dt, order, device="cuda", ckpt_every="auto",
-                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK,
+                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS, time_block="auto",
                  tile_cols=0):
         super().__init__()
         self.device = torch.device(device)
@@ -113,13 +128,15 @@ class Time2dBatch(torch.nn.Module):
             shots = self._shots(sx, sy, ry)
             models = [(self.nx, self.ny, self.order)]
             held = plan_bytes(models, self._plan.shots, self.nt, self._plan.K) if self._plan else 0
+            tb = _resolve_time_block(self.time_block, len(shots) * (self.nx + self.order) * (self.ny + self.order),
+                                     self.ckpt_every)
             K = _choose_ckpt(self.ckpt_every, plan_bytes(models, shots, self.nt, 0), self.nt,
-                             self.device, self.hbm_fraction, held, self.time_block)
+                             self.device, self.hbm_fraction, held, tb)
             self._plan = None
             torch.cuda.empty_cache()
             self._plan = ShotPlan(self.order, self.nt, self.h, self.dt, [(self.nx, self.ny)], shots, K,
                                   self.device, tile_rows=self.tile_rows, graph_steps=self.graph_steps,
-                                  time_block=self.time_block, tile_cols=self.tile_cols)
+                                  time_block=tb, tile_cols=self.tile_cols)
             self._key = key
         self._plan.models[0].load(vel)
         return self._plan
@@ -174,7 +191,7 @@ class Time2dWindows(torch.nn.Module):
     """
 
     def __init__(self, nx, ny, h, w, dt, order, windows, device="cuda", ckpt_every="auto",
-                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK,
+                 hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS, time_block="auto",
                  tile_cols=0):
         super().__init__()
         self.device = torch.device(device)
@@ -191,6 +208,7 @@ class Time2dWindows(torch.nn.Module):
         models = [(q["x1"] - q["x0"], ny) for q in self.windows]
         shots = [dict(model=i, sx=q["sx"], sy=q["sy"], ry=q["ry"], rx0=q["rx0"], nrec=q["nrec"],
                       w=q.get("w", self.w)) for i, q in enumerate(self.windows)]
+        time_block = _resolve_time_block(time_block, sum((a + order) * (b + order) for a, b in models), ckpt_every)
         K = _choose_ckpt(ckpt_every, plan_bytes([(a, b, order) for a, b in models], shots, self.nt, 0),
                          self.nt, self.device, hbm_fraction, time_block=time_block)
         self.plan = ShotPlan(order, self.nt, h, dt, models, shots, K, self.device, tile_rows=tile_rows,

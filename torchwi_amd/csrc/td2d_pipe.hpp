@@ -20,7 +20,9 @@
 // neighbours of the centre row come from the adjacent lanes through DPP
 // wavefront shifts.  LDS carries only what moves between levels: the newest
 // row of each level (two slots), the rows of level k-2 that the next level
-// uses as its u_prev (read five and six iterations later: seven slots), and
+// uses as its u_prev (read five iterations later: six slots; the row above,
+// which a bottom Keys cell also needs, is the u_prev the wave read one
+// iteration earlier and kept), and
 // in the adjoint the gradient sum after each level (six slots), so the
 // gradient is read and written once per launch and accumulated in time order.
 //
@@ -47,9 +49,10 @@ struct TArgs {
 constexpr int PLANES = 64;  // float4 columns per strip (one wavefront)
 constexpr int PW = 10;      // register window slots (the row loop's unroll)
 constexpr int PQ = 5;       // register prefetch slots (divides PW)
-constexpr int PD = 4;       // prefetch distance in iterations (< PQ)
-constexpr int NPRV = 7;     // LDS slots of level k-2 rows (read 5 and 6 iterations after the write)
+constexpr int PD = 2;      // prefetch distance in iterations (< PQ)
+constexpr int NPRV = 6;     // LDS slots of level k-2 rows (read 5 iterations after the write)
 constexpr int NGRD = 6;     // LDS slots of gradient sums (read 5 iterations after the write)
+static_assert(NPRV == NGRD, "one slot counter serves both rings");
 
 template <int T, unsigned FLAGS>
 struct PLayout {
@@ -240,7 +243,8 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
 
     const int ET = ey1 ? dimy : E - 4 * T;
     const int nit = ET - O + 5 * T;
-    int c7 = 0, c6 = 0;   // iteration mod NPRV, mod NGRD
+    int c6 = 0;   // iteration mod 6 (NPRV = NGRD)
+    float4 Pk = make_float4(0.f, 0.f, 0.f, 0.f);   // u_prev read this iteration (level waves)
 
     auto body = [&](int i, auto UC) {
         constexpr int U = decltype(UC)::value;
@@ -248,24 +252,26 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
         if (k == 0) {
             // loader: rows O + i of both input fields (and the gradient)
             S[curW + (U % 2) * PLANES] = qB[U % PQ];
-            S[prvW + c7 * PLANES] = qA[U % PQ];
+            S[prvW + c6 * PLANES] = qA[U % PQ];
             if constexpr ((FLAGS & F_GRAD) != 0) S[grdW + c6 * PLANES] = qC[U % PQ];
         } else {
             // append row r+4 of level k-1 (written by wave k-1 last iteration)
             win[U] = S[curR + ((U + 1) % 2) * PLANES];
+            // u_prev of row r (level k-2, written by wave k-1 five iterations
+            // ago); P1 keeps the one of row r-1
+            const float4 P1 = Pk;
+            const int c6r = c6 + 1 >= NPRV ? c6 + 1 - NPRV : c6 + 1;   // (i-5) mod 6
+            Pk = S[prvR + c6r * PLANES];
             const int r = O + i - 5 * k;
             if (active && r >= Ok && r < Ek) {
                 const float4 C = win[(U + PW - 4) % PW];
-                const int c7r = c7 + 2 >= NPRV ? c7 + 2 - NPRV : c7 + 2;   // (i-5) mod 7
-                const float4 P = S[prvR + c7r * PLANES];
+                const float4 P = Pk;
                 const float4 M = qA[U % PQ];
                 float4 un;
                 if (r < NE) {
                     un = make_float4(0.f, 0.f, 0.f, 0.f);
                 } else if (r >= dimy - NE) {
                     // bottom Keys rows: every cell, corners included
-                    const int c7r1 = c7 + 1 >= NPRV ? c7 + 1 - NPRV : c7 + 1;   // (i-6) mod 7
-                    const float4 P1 = S[prvR + c7r1 * PLANES];
                     const float4 C1 = win[(U + PW - 5) % PW], C2 = win[(U + PW - 6) % PW];
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
@@ -384,7 +390,6 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
                     }
                 }
                 if constexpr ((FLAGS & F_GRAD) != 0) {
-                    const int c6r = c6 + 1 >= NGRD ? c6 + 1 - NGRD : c6 + 1;   // (i-5) mod 6
                     const float4 gin = S[grdR + c6r * PLANES];
                     const float4 v = qB[U % PQ];
                     float4 g;
@@ -400,11 +405,10 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
                 }
                 if (writes_next) {
                     S[curW + (U % 2) * PLANES] = un;
-                    S[prvW + c7 * PLANES] = C;
+                    S[prvW + c6 * PLANES] = C;
                 }
             }
         }
-        c7 = c7 + 1 == NPRV ? 0 : c7 + 1;
         c6 = c6 + 1 == NGRD ? 0 : c6 + 1;
         __syncthreads();
     };

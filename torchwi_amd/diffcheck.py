@@ -31,7 +31,7 @@ from scipy import ndimage
 
 from . import Time2dBatch
 from . import workloads as W
-from .shots import default_ckpt_interval
+from .shots import auto_time_block, default_ckpt_interval
 
 
 def direction(nx, ny, seed, sigma=20.0, margin=10):
@@ -67,6 +67,10 @@ def main(argv=None):
     ap.add_argument("--eps", type=float, default=1e-3)
     ap.add_argument("--ckpt-every", type=int, default=0)
     ap.add_argument("--no-full", action="store_true", help="skip the full-storage comparison")
+    ap.add_argument("--time-block", type=int, default=0,
+                    help="steps per launch (1: one step; 2..4: temporally blocked kernel; 0: automatic by size)")
+    ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--tile-cols", type=int, default=0)
     args = ap.parse_args(argv)
     dev = torch.device("cuda:0")
     c = W.dipping_1000x3000(nshots=args.shots, nt=args.nt)
@@ -76,18 +80,20 @@ def main(argv=None):
         c.update(nx=args.nx, ny=args.nz, vel=W.to_user(v),
                  sx=[float(W.cell_pos(int(x), c["h"])) for x in xs])
     order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
-    K = args.ckpt_every or default_ckpt_interval(nt)
+    tb = args.time_block or auto_time_block(args.shots * (nx + order) * (ny + order))
+    K = args.ckpt_every or default_ckpt_interval(nt, tb)
+    kcfg = dict(time_block=tb, tile_rows=args.tile_rows, tile_cols=args.tile_cols)
     w = torch.from_numpy(c["w"])
     vel = torch.from_numpy(c["vel"]).to(dev)
     pert = direction(nx, ny, seed=99, sigma=30.0)
     vtrue = vel * (1.0 + 0.02 * torch.from_numpy(pert).to(dev))
     out = {"workload": "%dx%d model, h=%g m, dt=%g s, nt=%d, order %d, %d ricker(6) shots, %d receivers"
-                       % (ny, nx, h, dt, nt, order, args.shots, nx), "ckpt_every": K}
+                       % (ny, nx, h, dt, nt, order, args.shots, nx), "ckpt_every": K, "kernel": kcfg}
 
     # 1. shot 0 with full storage (before the batch plan: both would not fit)
     g0_full = d0 = None
     if not args.no_full:
-        mf = Time2dBatch(nx, ny, h, w, dt, order, dev, ckpt_every=0)
+        mf = Time2dBatch(nx, ny, h, w, dt, order, dev, ckpt_every=0, **kcfg)
         one = (c["sx"][:1], c["sy"][:1], c["ry"][:1])
         d0 = mf.modeling(vtrue, *one).clone()
         t0 = time.perf_counter()
@@ -100,7 +106,7 @@ def main(argv=None):
         del mf
         torch.cuda.empty_cache()
 
-    m = Time2dBatch(nx, ny, h, w, dt, order, dev, ckpt_every=K)
+    m = Time2dBatch(nx, ny, h, w, dt, order, dev, ckpt_every=K, **kcfg)
     t0 = time.perf_counter()
     d_obs = m.modeling(vtrue, c["sx"], c["sy"], c["ry"]).clone()
     torch.cuda.synchronize()

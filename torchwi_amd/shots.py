@@ -20,6 +20,20 @@ from . import _lib
 ALIGN = 32  # row stride alignment in floats (128 B)
 GRAPH_STEPS = 100  # time steps per captured hipGraph block (0: direct launches)
 TIME_BLOCK = 1     # time steps per launch of the step kernel (2..4: temporal blocking)
+# Cells per time step (all shots of a plan) from which the temporally blocked
+# kernel with 3 steps per launch is the default.  Measured on MI355X with
+# checkpointed gradients: below, the one-step kernel is faster because the
+# three wavefields of all shots largely stay in the 256 MB Infinity Cache
+# (32 shots of 1032 x 392 = 13M cells: 193 vs 128 Gcell/s; 128 shots, 52M
+# cells: 153 vs 130); above, the blocked kernel moves fewer HBM bytes and wins
+# (256 shots, 104M cells: 194 vs 147; 64 shots of 3008 x 1008, 194M cells:
+# 236 vs 181; 256 uneven windows, ~200M cells: picked by the autotuner).
+AUTO_BLOCK_CELLS = 75_000_000
+
+
+def auto_time_block(cells_per_step):
+    """Steps per launch for a plan advancing cells_per_step cells per step."""
+    return 3 if cells_per_step >= AUTO_BLOCK_CELLS else 1
 RING = 4           # fields per wavefield ring (3 rotate for one-step launches; 2 pairs for blocks)
 
 
