@@ -147,20 +147,35 @@ class ShotPlan:
             acc += fs[s]
             racc += self.nt * nrec[s]
         nf = acc
-        e = lambda n: torch.empty(n, dtype=torch.float32, device=dev)  # noqa: E731
         self.forward_only = bool(forward_only)
         if self.forward_only and self.K:
             raise ValueError("forward_only plans take ckpt_every = 0")
-        self.u = torch.zeros(RING * nf, dtype=torch.float32, device=dev)
-        self.frd = torch.zeros(racc, dtype=torch.float32, device=dev)
-        if self.forward_only:
-            self.lam = self.virt = self.ckpt = self.grad = self.resid = None
-        else:
-            self.lam = torch.zeros(RING * nf, dtype=torch.float32, device=dev) if self.K else self.u
-            self.virt = e(slots * nf)
-            self.ckpt = e(2 * self.nseg * nf) if self.K else None
-            self.grad = torch.zeros(nf, dtype=torch.float32, device=dev)
-            self.resid = torch.zeros(racc, dtype=torch.float32, device=dev)
+        # Every buffer of the plan is a 2 MiB-aligned slice of one device
+        # allocation, so the plan's working set is one contiguous virtual
+        # range.  Measured on MI355X against one allocation per buffer: the
+        # checkpointed gradient of 64 shots of 3008 x 1008 (215 GB) took
+        # 12.05 s instead of 13.1 s, the full-storage gradient of 32 shots of
+        # 1032 x 392 381 ms instead of 390 ms.
+        sizes = [("u", RING * nf, True), ("frd", racc, True)]
+        if not self.forward_only:
+            if self.K:
+                sizes.append(("lam", RING * nf, True))
+            sizes += [("virt", slots * nf, False), ("grad", nf, True), ("resid", racc, True)]
+            if self.K:
+                sizes.append(("ckpt", 2 * self.nseg * nf, False))
+        aln = (2 << 20) // 4
+        total = sum(round_up(n, aln) for _, n, _ in sizes)
+        self.arena = torch.empty(total, dtype=torch.float32, device=dev)
+        self.lam = self.virt = self.ckpt = self.grad = self.resid = None
+        off = 0
+        for name, n, zero in sizes:
+            t = self.arena[off:off + n]
+            if zero:
+                t.zero_()
+            setattr(self, name, t)
+            off += round_up(n, aln)
+        if not self.forward_only and not self.K:
+            self.lam = self.u
         self._wavelets = []
         spec = _lib.TwPlanSpec(order=order, nt=self.nt, h=self.h, dt=self.dt, ckpt_every=self.K,
                                tile_rows=int(tile_rows), graph_steps=int(graph_steps),
