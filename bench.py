@@ -12,7 +12,10 @@ perturbation (untimed).  Inputs are resident in HBM before timing starts.
 Throughput is Gcell-updates/s with one cell-update = one padded-grid cell of
 one wavefield advanced one step for one shot; a gradient step is
 2 * shots * dimx * dimy * nt cell-updates (checkpoint recomputation is not
-credited).  Prints one JSON line.
+credited).  Prints one JSON line.  Besides the headline (full storage, which
+fits in HBM here), the line carries the same workload checkpointed, the plain
+one-shot-per-launch form, and the checkpointed gradient of the 1000 x 3000
+model with 64 shots (nt = 8000), where checkpointing is required.
 """
 import argparse
 import json
@@ -55,6 +58,8 @@ def parse():
                     help="time steps per captured hipGraph block (0: one direct launch per step)")
     ap.add_argument("--compare", type=int, default=1,
                     help="also time the checkpointed form and the plain one-shot-per-launch form")
+    ap.add_argument("--no-scale", action="store_true",
+                    help="skip the checkpointed gradient of the 1000x3000 model with 64 shots (215 GB)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="first time a device-to-device copy of MB megabytes (achievable-bandwidth "
                          "reference and PMC byte-counter calibration)")
@@ -185,6 +190,41 @@ def run_plain(c, nshots, dev):
             "s_per_shot": round(el / nshots, 4)}
 
 
+def run_scale(dev):
+    """Checkpointed misfit gradient of the 1000 x 3000 model (h = 10 m,
+    dt = 0.8 ms, nt = 8000, order 8, 64 ricker(6) shots, 3000 receivers),
+    where full storage (97 GB per shot) cannot be used: one untimed modelling
+    of the observed data, then one timed gradient.  The step kernel is chosen
+    by plan size as Time2dBatch does (the temporally blocked kernel here)."""
+    import torch
+    from torchwi_amd import Time2dBatch
+    from torchwi_amd import workloads as W
+    from torchwi_amd.shots import auto_time_block, default_ckpt_interval
+    c = W.dipping_1000x3000(nshots=64, nt=8000)
+    order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
+    S = len(c["sx"])
+    tb = auto_time_block(S * (nx + order) * (ny + order))
+    K = default_ckpt_interval(nt, tb)
+    m = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K, time_block=tb)
+    vel = torch.from_numpy(c["vel"]).to(dev)
+    d_obs = m.modeling(vel * 1.01, c["sx"], c["sy"], c["ry"]).clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    cells = 2.0 * S * (nx + order) * (ny + order) * nt
+    v = cells / t / 1e9
+    del m, d_obs
+    torch.cuda.empty_cache()
+    return {"workload": "checkpointed misfit gradient, %d shots, %dx%d model (padded %dx%d), h=%g m, dt=%g s, "
+                        "nt=%d, order %d, checkpoint every %d steps + recompute, %d steps per launch"
+                        % (S, ny, nx, ny + order, nx + order, h, dt, nt, order, K, tb),
+            "value": round(v, 3), "unit": "Gcell-updates/s", "s_per_gradient": round(t, 3),
+            "gradient_shots_per_s": round(S / t, 3), "ckpt_every": K, "time_block": tb,
+            "effective_hbm_frac": round(v * 1e9 * (BYTES_GRAD_STEP / 2.0) / (HBM_PEAK_GBS * 1e9), 4)}
+
+
 def load_traffic(mode_key, kernel_key):
     """HBM bytes per launch of a step kernel at this workload, from the
     calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs kept in profiles/
@@ -269,6 +309,8 @@ def main():
         out["plain_one_shot"] = pl
         out["speedup_vs_plain"] = {"best": round(r["shots_per_s"] / pl["gradient_shots_per_s"], 2),
                                    "checkpointed": round(ck_rate / pl["gradient_shots_per_s"], 2)}
+    if not args.no_scale:
+        out["checkpointed_at_scale"] = run_scale(dev)
     if probe:
         out["copy_probe"] = probe
     if not args.no_cpu_baseline:
