@@ -284,10 +284,13 @@ def test_graph_replay_matches_direct_launches(K):
         assert loss == out[0][2]
 
 
-def test_windows_match_oracle_on_cropped_models():
+@pytest.mark.parametrize("time_block,K", [(1, 0), (3, 12)])
+def test_windows_match_oracle_on_cropped_models(time_block, K):
     """Shots on uneven aperture windows of one model, in one plan: each gather
     and gradient equals the oracle on the cropped sub-model (the reference's
-    Time2d on that sub-model), and the full gradient is their windowed sum."""
+    Time2d on that sub-model), and the full gradient is their windowed sum;
+    with one step per launch and with the blocked kernel (3 steps) over
+    checkpoints."""
     from torchwi_amd import Time2dWindows
     order, nx, ny, nt, h, dt = 8, 260, 40, 150, 10.0, 1e-3
     rng = np.random.default_rng(31)
@@ -299,7 +302,8 @@ def test_windows_match_oracle_on_cropped_models():
         nxw = x1 - x0
         wins.append(dict(x0=x0, x1=x1, sx=(nxw // 2 + 0.5) * h, sy=2.5 * h, ry=2.5 * h,
                          rx0=5, nrec=nxw - 10))
-    m = Time2dWindows(nx, ny, h, torch.from_numpy(w), dt, order, wins, DEV, ckpt_every=0)
+    print(f"case windows time_block={time_block} K={K}", flush=True)
+    m = Time2dWindows(nx, ny, h, torch.from_numpy(w), dt, order, wins, DEV, ckpt_every=K, time_block=time_block)
     velt = torch.from_numpy(vel).to(DEV)
     frds = [f.clone() for f in m.modeling(velt)]
     d_obs = [0.7 * f for f in frds]
@@ -386,3 +390,34 @@ def test_forward_only_plan_and_autotune():
                            [dict(q, w=None) for q in shots], DEV, nt_probe=30)
     assert best in [dict(x) for x in CANDIDATES]
     assert len(times) == len(CANDIDATES)
+
+
+def test_auto_time_block_large_plan_bitexact():
+    """A plan above shots.AUTO_BLOCK_CELLS cells per step runs the blocked
+    kernel by default (3 steps per launch, many strips and chunks per shot)
+    and gives the gathers and per-shot gradients of one step per launch."""
+    from torchwi_amd import Time2dBatch
+    from torchwi_amd.shots import AUTO_BLOCK_CELLS
+    order, nx, ny, nt, h, dt, S = 8, 1024, 912, 24, 10.0, 1e-3, 80
+    assert S * (nx + order) * (ny + order) >= AUTO_BLOCK_CELLS
+    rng = np.random.default_rng(5)
+    vel = torch.from_numpy((1500 + 2500 * rng.random((nx, ny))).astype(np.float32)).to(DEV)
+    t = np.arange(nt) * dt
+    w = torch.from_numpy(((1 - 2 * (np.pi * 25 * (t - 0.01)) ** 2) *
+                          np.exp(-(np.pi * 25 * (t - 0.01)) ** 2)).astype(np.float32))
+    sx = [(13 * s % nx + 0.5) * h for s in range(S)]
+    sy = [(s % 7 + 0.5) * h for s in range(S)]
+    ry = [(s % 5 + 0.5) * h for s in range(S)]
+    print(f"case auto time block S={S} nx={nx} ny={ny} nt={nt}", flush=True)
+    res = []
+    for tb in ("auto", 1):
+        m = Time2dBatch(nx, ny, h, w, dt, order, DEV, ckpt_every=0, time_block=tb)
+        frd = m.modeling(vel, sx, sy, ry).clone()
+        m.misfit_gradient(vel, sx, sy, ry, 0.5 * frd)
+        res.append((m.plan.time_block, frd, [m.plan.grad_of(s).clone() for s in (0, S // 2, S - 1)]))
+        del m
+        torch.cuda.empty_cache()
+    assert res[0][0] == 3 and res[1][0] == 1
+    assert torch.equal(res[0][1], res[1][1])
+    for a, b in zip(res[0][2], res[1][2]):
+        assert torch.equal(a, b)
