@@ -8,8 +8,8 @@
 // of halo above and below (again except at grid edges).  It marches down the
 // rows, one row per wave per iteration, with one wave per time level:
 //
-//   wave 0 (loader) streams the two input fields (levels -1 and 0), and in
-//          the adjoint the gradient, from HBM into LDS, PD rows ahead;
+//   wave 0 (loader) streams the two input fields (levels -1 and 0) from HBM
+//          into LDS, PD rows ahead;
 //   wave k (level k) at iteration i computes level k of row r = O + i - 5k.
 //
 // Level k lags level k-1 by five rows (the stencil radius, 4, plus one), so
@@ -23,7 +23,8 @@
 // uses as its u_prev (read five iterations later: six slots; the row above,
 // which a bottom Keys cell also needs, is the u_prev the wave read one
 // iteration earlier and kept), and
-// in the adjoint the gradient sum after each level (six slots), so the
+// in the adjoint the gradient sum after each level but the last (six slots;
+// wave 1 loads the stored gradient itself), so the
 // gradient is read and written once per launch and accumulated in time order.
 //
 // The halo of a level is valid where its dependency cone stays inside the
@@ -58,8 +59,8 @@ template <int T, unsigned FLAGS>
 struct PLayout {
     static constexpr int CUR = 0;                          // ring k: 2 rows of level k
     static constexpr int PRV = CUR + T * 2 * PLANES;       // ring k: NPRV rows of level k-1
-    static constexpr int GRD = PRV + T * NPRV * PLANES;    // ring k: NGRD rows of gradient sums
-    static constexpr int KS = GRD + ((FLAGS & F_GRAD) != 0 ? T * NGRD * PLANES : 0);  // per level: Keys scratch
+    static constexpr int GRD = PRV + T * NPRV * PLANES;    // ring k-1: NGRD rows of gradient sums after level k
+    static constexpr int KS = GRD + ((FLAGS & F_GRAD) != 0 ? (T - 1) * NGRD * PLANES : 0);  // per level: Keys scratch
     static constexpr int SIZE = KS + T * 32;
 };
 
@@ -177,6 +178,10 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
     // when the value will not be used), so the in-order count of outstanding
     // vector-memory operations is the same on every path and the wait for one
     // queue slot never waits for the slots behind it.
+    // The adjoint keeps three queues live (model, virtual source, and the
+    // gradient rows wave 1 starts from) and prefetches one iteration ahead;
+    // the other variants two.
+    constexpr int pd = PD;
     float4 qA[PQ], qB[PQ], qC[PQ];
     auto issue = [&](int slot_i, float4& A, float4& B, float4& C) {
         const int row = O + slot_i - 5 * k;
@@ -185,7 +190,7 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
         const int ib = (useB && bLane && rowB) ? row * ld + gx : 0;
         if constexpr ((FLAGS & F_GRAD) != 0) {
             B = k > 0 ? gld4_nt(pB + ib) : gld4(pB + ib);
-            const bool cok = k == 0 && lcore && row >= Y0 && row < Y1;
+            const bool cok = k == 1 && lcore && row >= Y0 && row < Y1;
             C = gld4(d.grad + (cok ? row * ld + gx : 0));
         } else {
             B = gld4(pB + ib);
@@ -230,8 +235,8 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
     const int curW = LY::CUR + k * 2 * PLANES + lane;
     const int prvR = LY::PRV + (k > 0 ? k - 1 : 0) * NPRV * PLANES + lane;
     const int prvW = LY::PRV + k * NPRV * PLANES + lane;
-    const int grdR = LY::GRD + (k > 0 ? k - 1 : 0) * NGRD * PLANES + lane;
-    const int grdW = LY::GRD + k * NGRD * PLANES + lane;
+    const int grdR = LY::GRD + (k > 1 ? k - 2 : 0) * NGRD * PLANES + lane;   // sum after level k-1
+    const int grdW = LY::GRD + (k > 0 ? k - 1 : 0) * NGRD * PLANES + lane;   // sum after level k
     const bool writes_next = k < T && k < a.nsteps;
     const int ksb = LY::KS + (k > 0 ? k - 1 : 0) * 32;   // this level's Keys scratch (float4 index)
 
@@ -239,7 +244,7 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
 #pragma unroll
     for (int q = 0; q < PW; ++q) win[q] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-    for (int q = 0; q < PD; ++q) issue(q, qA[q], qB[q], qC[q]);
+    for (int q = 0; q < pd; ++q) issue(q, qA[q], qB[q], qC[q]);
 
     const int ET = ey1 ? dimy : E - 4 * T;
     const int nit = ET - O + 5 * T;
@@ -248,12 +253,11 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
 
     auto body = [&](int i, auto UC) {
         constexpr int U = decltype(UC)::value;
-        issue(i + PD, qA[(U + PD) % PQ], qB[(U + PD) % PQ], qC[(U + PD) % PQ]);
+        issue(i + pd, qA[(U + pd) % PQ], qB[(U + pd) % PQ], qC[(U + pd) % PQ]);
         if (k == 0) {
             // loader: rows O + i of both input fields (and the gradient)
             S[curW + (U % 2) * PLANES] = qB[U % PQ];
             S[prvW + c6 * PLANES] = qA[U % PQ];
-            if constexpr ((FLAGS & F_GRAD) != 0) S[grdW + c6 * PLANES] = qC[U % PQ];
         } else {
             // append row r+4 of level k-1 (written by wave k-1 last iteration)
             win[U] = S[curR + ((U + 1) % 2) * PLANES];
@@ -390,7 +394,9 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
                     }
                 }
                 if constexpr ((FLAGS & F_GRAD) != 0) {
-                    const float4 gin = S[grdR + c6r * PLANES];
+                    // wave 1 starts from the stored gradient (its own queue);
+                    // wave k > 1 from wave k-1's sum, written five iterations ago
+                    const float4 gin = k == 1 ? qC[U % PQ] : S[grdR + c6r * PLANES];
                     const float4 v = qB[U % PQ];
                     float4 g;
                     g.x = gin.x + (-v.x) * un.x;
