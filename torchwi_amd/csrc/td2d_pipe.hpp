@@ -266,6 +266,12 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
             const float4 P1 = Pk;
             const int c6r = c6 + 1 >= NPRV ? c6 + 1 - NPRV : c6 + 1;   // (i-5) mod 6
             Pk = S[prvR + c6r * PLANES];
+            // gradient sum after level k-1 (wave k-1, five iterations ago);
+            // read as one 16-byte LDS access on every iteration (wave 1 reads
+            // ring 0 and discards it), not inside the selection below, where it
+            // would become four conditional 4-byte reads
+            float4 Gk = make_float4(0.f, 0.f, 0.f, 0.f);
+            if constexpr ((FLAGS & F_GRAD) != 0) Gk = S[grdR + c6r * PLANES];
             const int r = O + i - 5 * k;
             if (active && r >= Ok && r < Ek) {
                 const float4 C = win[(U + PW - 4) % PW];
@@ -394,9 +400,9 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
                     }
                 }
                 if constexpr ((FLAGS & F_GRAD) != 0) {
-                    // wave 1 starts from the stored gradient (its own queue);
-                    // wave k > 1 from wave k-1's sum, written five iterations ago
-                    const float4 gin = k == 1 ? qC[U % PQ] : S[grdR + c6r * PLANES];
+                    // wave 1 starts from the stored gradient (its own queue),
+                    // wave k > 1 from wave k-1's sum
+                    const float4 gin = k == 1 ? qC[U % PQ] : Gk;
                     const float4 v = qB[U % PQ];
                     float4 g;
                     g.x = gin.x + (-v.x) * un.x;
