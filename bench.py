@@ -52,6 +52,8 @@ def parse():
                     help="time steps per launch (2..4: temporally blocked kernel)")
     ap.add_argument("--tile-rows", type=int, default=0,
                     help="tile height (one step per launch: 32 or 64; blocked kernel: rows per chunk); 0 automatic")
+    ap.add_argument("--grad-group", type=int, default=1,
+                    help="shots per gradient group of the one-step adjoint (1: one gradient per shot; > 1 is refused by the library for now)")
     ap.add_argument("--tile-cols", type=int, default=0,
                     help="one-step kernel tile width: 64 or 128; 0 automatic")
     ap.add_argument("--graph-steps", type=int, default=100,
@@ -118,7 +120,8 @@ def cpu_baseline():
                       "forward with full storage + adjoint gradient; median of 5 runs (%.3f s each)" % t}
 
 
-def run_batched(c, K, steps, warmup, timing, graph_steps, dev, time_block=1, tile_rows=0, tile_cols=0):
+def run_batched(c, K, steps, warmup, timing, graph_steps, dev, time_block=1, tile_rows=0, tile_cols=0,
+                grad_group=1):
     """Time `steps` misfit gradients of the batched engine; returns stats."""
     import torch
     from torchwi_amd import Time2dBatch
@@ -126,7 +129,7 @@ def run_batched(c, K, steps, warmup, timing, graph_steps, dev, time_block=1, til
     S = len(c["sx"])
     model = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K,
                         graph_steps=graph_steps, time_block=time_block, tile_rows=tile_rows,
-                        tile_cols=tile_cols)
+                        tile_cols=tile_cols, grad_group=grad_group)
     vel = torch.from_numpy(c["vel"]).to(dev)
     rng = np.random.default_rng(7)
     vtrue = torch.from_numpy(c["vel"] * (1.0 + 0.02 * rng.standard_normal(c["vel"].shape)).astype(np.float32)).to(dev)
@@ -255,8 +258,9 @@ def main():
     tb = args.time_block
     kck = args.ckpt_every or default_ckpt_interval(nt, tb)
     K = {"full": 0, "ckpt": kck, "auto": "auto"}[args.mode]
+    gg = args.grad_group if tb == 1 else 1
     r = run_batched(c, K, args.steps, args.warmup, not args.no_timing, args.graph_steps, dev, tb, args.tile_rows,
-                    args.tile_cols)
+                    args.tile_cols, gg)
     S, kstate, value, elapsed, lat = r["S"], r["K"], r["value"], r["elapsed"], r["lat"]
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
@@ -270,7 +274,7 @@ def main():
                                   "full virtual-source storage" if kstate == 0 else "checkpoint every %d steps + recompute" % kstate),
                    "shots": S, "nz": ny, "nx": nx, "nt": nt, "order": order, "ckpt_every": kstate,
                    "graph_steps": args.graph_steps, "time_block": tb, "tile_rows": args.tile_rows,
-                   "tile_cols": args.tile_cols},
+                   "tile_cols": args.tile_cols, "grad_group": gg},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "gradient_shots_per_s": round(r["shots_per_s"], 3),
         "effective_hbm_frac": round(value * 1e9 * (BYTES_GRAD_STEP / 2.0) / (HBM_PEAK_GBS * 1e9), 4),
@@ -290,14 +294,16 @@ def main():
         b = {"fwd": BYTES_FWD, "fwd_virt": BYTES_FWD_VIRT, "recomp": BYTES_FWD_VIRT, "adj": BYTES_ADJ}[dom]
         ach = per[dom]["achieved_GBs"]
         tr = (load_traffic("full_storage" if kstate == 0 else "checkpointed", dom)
-              if (nt, S, order, tb, args.tile_rows, args.tile_cols) == (4000, 32, 8, 1, 0, 0) else None)
-        out["roofline"] = {"bound": "hbm", "kernel": ("k_pipe<4,%s,%d>" % (dom, tb)) if tb > 1 else "k_vstep<4,%s>" % dom,
+              if (nt, S, order, tb, args.tile_rows, args.tile_cols, gg) == (4000, 32, 8, 1, 0, 0, 1) else None)
+        kname = ("k_pipe<4,%s,%d>" % (dom, tb) if tb > 1 else
+                 "k_vadj_group<4> (%d shots per group)" % gg if dom == "adj" and gg > 1 else "k_vstep<4,%s>" % dom)
+        out["roofline"] = {"bound": "hbm", "kernel": kname,
                            "achieved": ach,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                            "traffic": tr, "bytes_per_cell": b, "cells_per_launch": cells_launch}
     if args.compare:
         if kstate == 0:
-            rc = run_batched(c, kck, 1, 0, False, args.graph_steps, dev, tb, args.tile_rows, args.tile_cols)
+            rc = run_batched(c, kck, 1, 0, False, args.graph_steps, dev, tb, args.tile_rows, args.tile_cols, gg)
             out["checkpointed"] = {"ckpt_every": rc["K"], "value": round(rc["value"], 3),
                                    "ms_per_step": round(rc["elapsed"] * 1e3, 3),
                                    "gradient_shots_per_s": round(rc["shots_per_s"], 3),

@@ -122,6 +122,11 @@ typedef struct TwPlanSpec {
                          adjoint ring (virt, lam, ckpt may be NULL; ckpt_every
                          must be 0; tw_plan_backward is refused) */
     int tile_cols;    /* one-step kernel tile width: 64 or 128, 0 = automatic */
+    int grad_group;   /* 0 or 1: one gradient per shot.  > 1 (an adjoint that
+                         accumulates the imaging condition of a group of shots
+                         of one model into one gradient) is currently refused
+                         with TW_EINVAL: its kernel faulted on MI355X for a
+                         cause not yet found */
 } TwPlanSpec;
 
 typedef struct TwShotSpec {

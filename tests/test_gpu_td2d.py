@@ -365,6 +365,63 @@ def test_temporal_blocking_bitexact(order, T, nx, ny, nt, K, rows):
     assert np.array_equal(res[1][0][1].cpu().numpy(), frd_o)
 
 
+@pytest.mark.skip(reason="grad_group > 1 is switched off: its first MI355X run (case 8-5-2-0-100) ended in a "
+                         "GPU memory fault whose cause has not been found; the plan refuses grad_group > 1")
+@pytest.mark.parametrize("order,S,G,K,graph", [
+    (8, 5, 2, 0, 100), (8, 8, 4, 12, 100), (4, 3, 4, 0, 0), (2, 6, 3, 7, 0), (8, 7, 7, 9, 0)])
+def test_grad_group_matches_per_shot(order, S, G, K, graph):
+    """With grad_group = G the adjoint steps G shots of the model through each
+    tile and accumulates their imaging condition into one gradient (the
+    group's first shot's buffer; the others stay 0).  Gathers and loss are
+    the per-shot plan's bit for bit; every group gradient equals the sum of
+    its shots' per-shot gradients, and the batch gradient the per-shot batch
+    gradient, to rounding (relative L2 <= 1e-6): only the order of the sum
+    over shots and time differs."""
+    from torchwi_amd import Time2dBatch
+    c = small_case(order, nx=90, ny=50, nt=120, seed=S * 11 + G)
+    h = c["h"]
+    sx = [(5 + 80 * i / max(S - 1, 1) + 0.5) * h for i in range(S)]
+    sy = [(i % 3 + 0.5) * h for i in range(S)]
+    ry = [(1 + i % 2 + 0.5) * h for i in range(S)]
+    print(f"case order={order} S={S} G={G} K={K} graph_steps={graph}", flush=True)
+    vel = torch.from_numpy(c["vel"]).to(DEV)
+    w = torch.from_numpy(c["w"])
+    res = []
+    for gg in (1, G):
+        m = Time2dBatch(c["nx"], c["ny"], h, w, c["dt"], order, DEV, ckpt_every=K, time_block=1,
+                        graph_steps=graph, grad_group=gg)
+        frd = m.modeling(vel, sx, sy, ry).clone()
+        loss, g = m.misfit_gradient(vel, sx, sy, ry, 0.6 * frd)
+        res.append((frd, float(loss), g.clone(), [m.plan.grad_of(s).clone() for s in range(S)]))
+    assert torch.equal(res[0][0], res[1][0]), "gathers"
+    assert res[0][1] == res[1][1], "loss"
+    assert rel_l2(res[1][2].cpu().numpy(), res[0][2].cpu().numpy()) < 1e-6
+    for s in range(S):
+        if s % G:
+            assert not torch.any(res[1][3][s] != 0), f"shot {s} is not a group leader"
+    for lead in range(0, S, G):
+        want = res[0][3][lead].clone()
+        for s in range(lead + 1, min(lead + G, S)):
+            want.add_(res[0][3][s])
+        assert rel_l2(res[1][3][lead].cpu().numpy(), want.cpu().numpy()) < 1e-6, f"group {lead}"
+
+
+def test_grad_group_refused():
+    """grad_group > 1 is refused with an error naming it (blocked kernel,
+    multi-row tiles, and while the group adjoint is switched off, every plan);
+    no kernel runs."""
+    from torchwi_amd.shots import ShotPlan
+    c = small_case(8, nx=40, ny=30, nt=30, seed=3)
+    w = torch.from_numpy(c["w"])
+    shots = [dict(model=0, sx=10.5 * c["h"], sy=2.5 * c["h"], ry=2.5 * c["h"], rx0=0, nrec=40, w=w)] * 2
+    with pytest.raises(RuntimeError, match="grad_group"):
+        ShotPlan(8, 30, c["h"], c["dt"], [(40, 30)], shots, 0, DEV, time_block=2, grad_group=2)
+    with pytest.raises(RuntimeError, match="grad_group"):
+        ShotPlan(8, 30, c["h"], c["dt"], [(40, 30)], shots, 0, DEV, tile_rows=32, grad_group=2)
+    with pytest.raises(RuntimeError, match="grad_group"):
+        ShotPlan(8, 30, c["h"], c["dt"], [(40, 30)], shots, 0, DEV, grad_group=2)
+
+
 def test_forward_only_plan_and_autotune():
     """A forward-only plan (no virtual-source storage) models the same gathers
     as a gradient plan and refuses the adjoint; the autotuner returns one of

@@ -45,7 +45,7 @@ KINDS = ("fwd", "fwd_virt", "recomp", "adj")
 class TwPlanSpec(ctypes.Structure):
     _fields_ = [("order", _ci), ("nt", _ci), ("h", _cf), ("dt", _cf), ("ckpt_every", _ci),
                 ("tile_rows", _ci), ("graph_steps", _ci), ("time_block", _ci), ("forward_only", _ci),
-                ("tile_cols", _ci)]
+                ("tile_cols", _ci), ("grad_group", _ci)]
 
 
 class TwShotSpec(ctypes.Structure):

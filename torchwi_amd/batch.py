@@ -90,13 +90,16 @@ class Time2dBatch(torch.nn.Module):
     ``time_block``: time steps per kernel launch; "auto" picks the
     temporally blocked kernel (3 steps) for plans of at least
     shots.AUTO_BLOCK_CELLS cells per step and one step per launch below.
+    ``grad_group``: shots per gradient group of the one-step adjoint
+    (shots.ShotPlan; values > 1 are refused for now); 1 keeps one gradient
+    per shot, whose sum in shot order is the returned gradient.
     The returned gathers are the plan's buffer: the next forward overwrites
     them (as the reference's module buffer, TimeFunction.py:128).
     """
 
     def __init__(self, nx, ny, h, w, dt, order, device="cuda", ckpt_every="auto",
                  hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS, time_block="auto",
-                 tile_cols=0):
+                 tile_cols=0, grad_group=1):
         super().__init__()
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -113,6 +116,7 @@ class Time2dBatch(torch.nn.Module):
         self.hbm_fraction = hbm_fraction
         self.tile_rows = tile_rows
         self.tile_cols = tile_cols
+        self.grad_group = int(grad_group)
         self._plan = None
         self._key = None
 
@@ -136,7 +140,8 @@ class Time2dBatch(torch.nn.Module):
             torch.cuda.empty_cache()
             self._plan = ShotPlan(self.order, self.nt, self.h, self.dt, [(self.nx, self.ny)], shots, K,
                                   self.device, tile_rows=self.tile_rows, graph_steps=self.graph_steps,
-                                  time_block=tb, tile_cols=self.tile_cols)
+                                  time_block=tb, tile_cols=self.tile_cols,
+                                  grad_group=self.grad_group if tb == 1 else 1)
             self._key = key
         self._plan.models[0].load(vel)
         return self._plan
@@ -146,11 +151,11 @@ class Time2dBatch(torch.nn.Module):
         return self._plan
 
     def reduce_grad(self, p):
-        """Sum the per-shot padded gradients in shot order, crop to the model
-        and transpose to (nx, ny) (TimeFunction.py:141-145)."""
+        """Sum the per-shot (or per-group) padded gradients in shot order,
+        crop to the model and transpose to (nx, ny) (TimeFunction.py:141-145)."""
         ne = self.ne
         tot = p.grad_of(0).clone()
-        for s in range(1, p.S):
+        for s in range(p.grad_group, p.S, p.grad_group):
             tot.add_(p.grad_of(s))
         return tot[ne:ne + self.ny, ne:ne + self.nx].transpose(0, 1)
 

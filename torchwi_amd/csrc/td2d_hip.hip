@@ -50,6 +50,7 @@ struct ShotDesc {
     int dimx, dimy, ld;
     int tiles_x, tile0;
     int mstrips, mchunks;     // temporally blocked kernel: column strips x row chunks
+    int gcount;               // shots in the gradient group this shot leads (0: not a leader)
     int rrow, rx0, nrec;      // receiver row / window in padded coordinates
     long long isrc;           // linear (ld-strided) index of the source cell
     long long fstride;        // floats between consecutive fields of a ring
@@ -629,9 +630,13 @@ struct Plan {
     long long max_field = 0;
     std::vector<ShotDesc> host;
     std::vector<int2> tiles;          // (shot, by << 16 | bx), dealing order
+    int ggroup = 1;                   // shots per gradient group of the one-step adjoint (1: per shot)
+    std::vector<int2> gtiles;         // (group leader, by << 16 | bx): work list of the group adjoint
+    int gntiles = 0, gper_xcd = 0;
     std::vector<ModelRef> models;
     ShotDesc* d_desc = nullptr;
     int2* d_tiles = nullptr;
+    int2* d_gtiles = nullptr;
     ~Plan()
     {
         g_fwd.release();
@@ -840,6 +845,16 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
             p.models.push_back({q.vel, q.vm, q.vc3, q.dimx, q.dimy, q.ld, d.dtoh2, d.cons});
         }
     }
+    for (ShotDesc& d : p.host) d.gcount = 1;
+    p.ggroup = spec->grad_group <= 1 ? 1 : spec->grad_group;
+    if (p.ggroup > 1 && (p.fwd_only || p.tb > 1))
+        return fail(TW_EINVAL, "grad_group > 1 needs a gradient plan with time_block 1");
+    // Switched off: the first MI355X run of the group adjoint (k_vadj_group)
+    // ended in a GPU memory fault whose cause has not been found (a CPU build
+    // of this file's plan code and kernels under AddressSanitizer runs the
+    // same plan cleanly and reproduces the per-shot gradient).
+    if (p.ggroup > 1)
+        return fail(TW_EINVAL, "grad_group > 1 is switched off (its adjoint kernel faulted on MI355X; cause not found)");
     if (p.tb > 1) return pipe_tiles(p, spec->tile_rows);
     if (spec->tile_cols != 0 && spec->tile_cols != 64 && spec->tile_cols != 128)
         return fail(TW_EINVAL, "tile_cols must be 0, 64 or 128 (got %d)", spec->tile_cols);
@@ -870,12 +885,37 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
     }
     p.ntiles = (int)p.tiles.size();
     p.per_xcd = (p.ntiles + NXCD - 1) / NXCD;
+    if (p.ggroup > 1) {
+        if (p.ryv != 1)
+            return fail(TW_EINVAL, "grad_group > 1 needs a one-row tile (64 x 16 or 128 x 8)");
+        // gradient groups: runs of at most ggroup consecutive shots on one model
+        int lead = 0;
+        for (int s = 0; s < nshots; ++s) {
+            if (s > 0 && p.host[s].vel == p.host[lead].vel && s - lead < p.ggroup) {
+                p.host[lead].gcount++;
+                p.host[s].gcount = 0;
+            } else {
+                lead = s;
+            }
+        }
+        for (const ModelRef& m : p.models) {
+            const int ntx = (m.dimx + tw - 1) / tw, nty = (m.dimy + ty - 1) / ty;
+            for (int by = 0; by < nty; ++by)
+                for (int bx = 0; bx < ntx; ++bx)
+                    for (int s = 0; s < nshots; ++s)
+                        if (p.host[s].vel == m.vel && p.host[s].gcount > 0)
+                            p.gtiles.push_back(make_int2(s, (by << 16) | bx));
+        }
+        p.gntiles = (int)p.gtiles.size();
+        p.gper_xcd = (p.gntiles + NXCD - 1) / NXCD;
+    }
     return TW_OK;
 }
 
 size_t plan_bytes(const Plan& p)
 {
-    return align256(sizeof(ShotDesc) * p.nshots) + align256(sizeof(int2) * p.ntiles) + 256;
+    return align256(sizeof(ShotDesc) * p.nshots) + align256(sizeof(int2) * p.ntiles) + 256 +
+           align256(sizeof(int2) * p.gntiles);
 }
 
 template <int NE, unsigned FLAGS, int CX4>
@@ -895,13 +935,24 @@ void launch_batch_nf(const Plan& p, const StepArgs& a, hipStream_t s)
 }
 
 template <int NE>
+void launch_group_ne(const Plan& p, const StepArgs& a, hipStream_t s)
+{
+    const dim3 g(p.gper_xcd * NXCD), b(VTHR);
+    if (p.cx4 == 32) hipLaunchKernelGGL((k_vadj_group<NE, 32>), g, b, 0, s, p.d_desc, p.d_gtiles, a);
+    else hipLaunchKernelGGL((k_vadj_group<NE, 16>), g, b, 0, s, p.d_desc, p.d_gtiles, a);
+}
+
+template <int NE>
 int launch_batch_ne(const Plan& p, const StepArgs& a, hipStream_t s)
 {
     switch (a.flags) {
     case M_FWD: launch_batch_nf<NE, M_FWD>(p, a, s); break;
     case M_FWD_VIRT: launch_batch_nf<NE, M_FWD_VIRT>(p, a, s); break;
     case M_RECOMP: launch_batch_nf<NE, M_RECOMP>(p, a, s); break;
-    case M_ADJ: launch_batch_nf<NE, M_ADJ>(p, a, s); break;
+    case M_ADJ:
+        if (p.ggroup > 1) launch_group_ne<NE>(p, a, s);
+        else launch_batch_nf<NE, M_ADJ>(p, a, s);
+        break;
     default: return fail(TW_EINVAL, "internal: no batch step variant for flags %u", a.flags);
     }
     return TW_OK;
@@ -925,6 +976,10 @@ int launch_batch(Plan& p, StepArgs a, hipStream_t s, bool timed = true)
 {
     a.ntiles = p.ntiles;
     a.per_xcd = p.per_xcd;
+    if (a.flags == M_ADJ && p.ggroup > 1) {
+        a.ntiles = p.gntiles;
+        a.per_xcd = p.gper_xcd;
+    }
     a.nt = p.nt;
     a.K = p.K;
     a.nseg = p.nseg;
@@ -1232,12 +1287,16 @@ int tw_plan_create(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, 
     p->d_tiles = (int2*)((char*)workspace + align256(sizeof(ShotDesc) * p->nshots));
     p->d_ctl = (int*)((char*)workspace + align256(sizeof(ShotDesc) * p->nshots) +
                       align256(sizeof(int2) * p->ntiles));
+    p->d_gtiles = (int2*)((char*)p->d_ctl + 256);
     p->graph_steps = spec->graph_steps;
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = hipMemcpyAsync(p->d_desc, p->host.data(), sizeof(ShotDesc) * p->nshots,
                                   hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
         e = hipMemcpyAsync(p->d_tiles, p->tiles.data(), sizeof(int2) * p->ntiles,
+                           hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && p->gntiles > 0)
+        e = hipMemcpyAsync(p->d_gtiles, p->gtiles.data(), sizeof(int2) * p->gntiles,
                            hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {

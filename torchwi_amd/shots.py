@@ -121,10 +121,16 @@ class ShotPlan:
     tile_rows, tile_cols: tile shape of the one-step kernel (64x32, 64x64,
             128x16, 128x32; 0 automatic) or, with time_block > 1, the row-chunk
             height of the blocked kernel.
+    grad_group: 1: one gradient per shot.  > 1 (the imaging condition of up
+            to this many consecutive shots of one model accumulated into one
+            gradient, the group's first shot's grad_of) is refused by the
+            library for now: its adjoint kernel faulted on MI355X for a cause
+            not yet found.
     """
 
     def __init__(self, order, nt, h, dt, models, shots, ckpt_every, device, tile_rows=0,
-                 graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK, forward_only=False, tile_cols=0):
+                 graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK, forward_only=False, tile_cols=0,
+                 grad_group=1):
         self.order, self.nt, self.h, self.dt = order, int(nt), float(h), float(dt)
         self.device = torch.device(device)
         self.K = int(ckpt_every)
@@ -180,8 +186,9 @@ class ShotPlan:
         spec = _lib.TwPlanSpec(order=order, nt=self.nt, h=self.h, dt=self.dt, ckpt_every=self.K,
                                tile_rows=int(tile_rows), graph_steps=int(graph_steps),
                                time_block=int(time_block), forward_only=int(self.forward_only),
-                               tile_cols=int(tile_cols))
+                               tile_cols=int(tile_cols), grad_group=int(grad_group))
         self.graph_steps = int(graph_steps)
+        self.grad_group = max(1, int(grad_group))
         self.time_block = int(time_block)
         cs = (_lib.TwShotSpec * S)()
         wcache = {}
@@ -233,7 +240,8 @@ class ShotPlan:
         return self.resid[self.roff[s]:self.roff[s] + self.nt * self._nrec[s]].view(self.nt, self._nrec[s])
 
     def grad_of(self, s):
-        """Padded-grid gradient of shot s, (dimy, ld)."""
+        """Padded-grid gradient of shot s, (dimy, ld) (with grad_group > 1:
+        of the group shot s leads, or zeros)."""
         m = self.models[self.shots[s]["model"]]
         return self.grad[self.foff[s]:self.foff[s] + self._fs[s]].view(m.dimy, m.ld)
 
