@@ -69,3 +69,50 @@ def test_plan_validation_errors_without_gpu():
     q.sx, q.sy, q.ry = 5000.0, 25.0, 25.0
     assert L.tw_plan_workspace_bytes(ctypes.byref(spec), shots, 1) == 0
     assert "source" in _lib.last_error()
+
+
+def header_struct_fields(name):
+    """Field names of `typedef struct name {...} name;` in include/*.h, in order."""
+    inc = os.path.join(ROOT, "include")
+    for f in sorted(os.listdir(inc)):
+        text = re.sub(r"/\*.*?\*/", "", open(os.path.join(inc, f)).read(), flags=re.S)
+        m = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), text, flags=re.S)
+        if m:
+            fields = []
+            for decl in m.group(1).split(";"):
+                decl = decl.strip()
+                if not decl:
+                    continue
+                parts = decl.split(None, 1) if not decl.startswith("const ") else ["const"] + decl[6:].split(None, 1)
+                for v in parts[-1].split(","):
+                    fields.append(v.strip().lstrip("*").strip())
+            return fields
+    raise AssertionError("struct %s not found" % name)
+
+
+@pytest.mark.parametrize("name", ["TwPlanSpec", "TwShotSpec"])
+def test_ctypes_structs_follow_the_header(name):
+    """The ctypes mirrors of the C-ABI structs have the header's fields in the
+    header's order (a binding that drifts would pass garbage silently)."""
+    from torchwi_amd import _lib
+    want = header_struct_fields(name)
+    got = [f for f, _ in getattr(_lib, name)._fields_]
+    assert got == want
+
+
+def test_grad_group_is_refused_on_the_host():
+    """grad_group > 1 is refused by plan validation (no device call)."""
+    from torchwi_amd import _lib
+    L = _lib.lib()
+    spec = _lib.TwPlanSpec(order=8, nt=10, h=10.0, dt=1e-3, ckpt_every=0, grad_group=2)
+    shots = (_lib.TwShotSpec * 2)()
+    for q in shots:
+        q.dimx, q.dimy, q.ld = 40, 30, 64
+        q.sx, q.sy, q.ry = 105.0, 25.0, 25.0
+        q.rx0, q.nrec = 0, 32
+        for p in ("vel", "vm", "vc3", "w", "u", "lam", "frd", "resid", "virt", "grad"):
+            setattr(q, p, 256)
+    assert L.tw_plan_workspace_bytes(ctypes.byref(spec), shots, 2) == 0
+    assert "grad_group" in _lib.last_error()
+    spec.grad_group = 1
+    assert L.tw_plan_workspace_bytes(ctypes.byref(spec), shots, 2) > 0
