@@ -258,7 +258,10 @@ int orc_backward_f32(float *grad, const float *virt, float *u1, float *u2, float
  * propagator_exa_cuda.cu:5-84).  The virtual source here is the EXA
  * kernel's own expression cons/(v*v*v) * ((u3 - 2u2) + u1).  Ties keep the
  * earliest step (strict >).  The reference exists only for CUDA, so this
- * function is a restatement with no runnable counterpart. */
+ * function has no runnable counterpart; it is pinned (tests/test_oracle.py)
+ * against the reference's full-storage virtual sources: on every cell whose
+ * largest source is clearly separated, the same step and the same signed
+ * value to rounding (the two kernels associate the sum differently). */
 int orc_exa_forward_f32(float *frd, float *exa, short *iexa, float *u1, float *u2,
                         float *u3, const float *v, const float *w, int order, int dimx,
                         int dimy, int nt, float h, float dt, float sx, float sy, float ry)
@@ -294,7 +297,9 @@ int orc_exa_forward_f32(float *frd, float *exa, short *iexa, float *u1, float *u
 
 /* EXA gradient: grad = -exa * lambda at the step recorded in iexa
  * (assignment, not accumulation) (td2d_cuda.cu:197-205,
- * propagator_exa_cuda.cu:87-159).  grad must be zeroed by the caller. */
+ * propagator_exa_cuda.cu:87-159).  grad must be zeroed by the caller.
+ * Pinned bit for bit against the reference's full-storage backward run with
+ * a virtual source that is exa at step iexa and zero at every other step. */
 int orc_exa_backward_f32(float *grad, const float *exa, const short *iexa, float *u1,
                          float *u2, float *u3, const float *v, const float *resid,
                          int order, int dimx, int dimy, int nt, float h, float dt,

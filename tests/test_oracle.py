@@ -229,3 +229,62 @@ def test_ricker_matches_reference_definition():
     spec = np.abs(np.fft.rfft(a, 1 << 16))
     fpk = np.argmax(spec) / (len(spec) - 1) * (0.5 / dt)
     assert abs(fpk - f / 2.16) < 0.05
+
+
+def _impl():
+    return "ref" if O.ref_available() else "oracle"
+
+
+@pytest.mark.parametrize("order", [2, 4, 8])
+def test_exa_forward_keeps_the_largest_virtual_source(order):
+    """EXA forward against the full-storage virtual sources of the reference's
+    CPU path: iexa[i] is the step of the largest-magnitude virtual source at
+    cell i and exa[i] that source, sign kept.  The two reference kernels sum
+    the three wavefields in different orders ((u3 - 2u2) + u1 in the EXA
+    kernel, td2d_cuda.cu:189; (u1 - 2u2) + u3 on the CPU,
+    propagator_cpu.cpp:71), a second difference that can cancel, so the
+    comparison is made on the cells whose largest magnitude is significant
+    and separated from the runner-up by more than 1e-3: there the step must
+    be the same and the value agree to 1e-3.  A wrong step index, a dropped
+    sign or a signed instead of absolute maximum fails this."""
+    c = small_case(order, nx=40, ny=26, nt=70, seed=order + 50)
+    vpad = O.pad_vel(c["vel"], order)
+    geo = (order, c["dimx"], c["dimy"], c["h"], c["dt"])
+    pos = (12.5 * c["h"], 3.5 * c["h"], 2.5 * c["h"])
+    frd_e, exa, iexa = O.exa_forward(vpad, c["w"], *geo, *pos)
+    frd, virt = O.forward(vpad, c["w"], *geo, *pos, impl=_impl())
+    assert np.array_equal(frd_e, frd)   # the same recursion
+    n = c["dimx"] * c["dimy"]
+    a = np.abs(virt.astype(np.float64))
+    order_t = np.sort(a, axis=0)
+    top, second = order_t[-1], order_t[-2]
+    t_max = np.argmax(a, axis=0)
+    clear = (top > 1e-6 * top.max()) & (top - second > 1e-3 * top)
+    assert np.count_nonzero(clear) > n // 4
+    assert np.array_equal(iexa[clear].astype(np.int64), t_max[clear]), "iexa is not the step of the largest source"
+    want = virt[t_max, np.arange(n)].astype(np.float64)
+    assert np.all(np.abs(exa[clear] - want[clear]) <= 1e-3 * top[clear]), "exa is not that source (value or sign)"
+    quiet = top == 0
+    assert np.all(exa[quiet] == 0) and np.all(iexa[quiet] == 0)
+
+
+@pytest.mark.parametrize("order", [2, 4, 8])
+def test_exa_gradient_is_the_imaging_condition_at_the_recorded_step(order):
+    """The EXA gradient -exa[i] * lambda_iexa[i][i] is the full-storage
+    imaging condition of a virtual source that is exa[i] at step iexa[i] and
+    0 at every other step: each other term adds a zero, so the reference's
+    full-storage backward reproduces it bit for bit.  An off-by-one pairing
+    of the recorded step with the adjoint field fails this."""
+    c = small_case(order, nx=40, ny=26, nt=70, seed=order + 60)
+    vpad = O.pad_vel(c["vel"], order)
+    geo = (order, c["dimx"], c["dimy"], c["h"], c["dt"])
+    _, exa, iexa = O.exa_forward(vpad, c["w"], *geo, 20.5 * c["h"], 2.5 * c["h"], 1.5 * c["h"])
+    rng = np.random.default_rng(order)
+    resid = rng.standard_normal((c["nt"], c["nx"])).astype(np.float32)
+    g_exa = O.exa_backward(exa, iexa, vpad, resid, *geo, 1.5 * c["h"])
+    n = c["dimx"] * c["dimy"]
+    sparse = np.zeros((c["nt"], n), np.float32)
+    sparse[iexa.astype(np.int64), np.arange(n)] = exa
+    g_full = O.backward(sparse, vpad, resid, *geo, 1.5 * c["h"], impl=_impl())
+    assert np.array_equal(g_exa, g_full)
+    assert np.count_nonzero(g_exa) > n // 10
