@@ -286,7 +286,9 @@ class RSF:
                 data.shape = self.shape
                 self.close()
         else:
-            data = np.loadtxt(self.f, dtype=self.dtype)
+            # the writer prints every type with a float format ('%.18e'), so
+            # integers are read as doubles and converted
+            data = np.loadtxt(self.f, dtype=np.float64 if self.type == 'int' else self.dtype).astype(self.dtype)
             if data.shape == ():
                 data.shape = (1,)
         return data
