@@ -243,8 +243,8 @@ def load_traffic(mode_key, kernel_key):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box of this build has one MI355X; "
-              "multi-GPU shot sharding is a design in DESIGN.md, not a run (--gpus %d refused)" % args.gpus,
+        print("bench.py measures one GPU (one MI355X); sharding shots over several GPUs is designed "
+              "(one gradient all-reduce per batch) but not run (--gpus %d refused)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -3,7 +3,7 @@
 The HIP kernels reproduce the reference CPU path's mixed-precision
 arithmetic, so the bar here is bit-exact equality with the oracle (which is
 itself bit-exact with the compiled reference, tests/test_oracle.py).  The
-north-star tolerance, relative L2 <= 1e-5, is asserted as well, and the FP32
+parity tolerance for floating point, relative L2 <= 1e-5, is asserted as well, and the FP32
 error against the FP64 run of the same recursion is printed per result.
 """
 import numpy as np
@@ -15,7 +15,7 @@ from tests.helpers import rel_l2, small_case
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-5  # relative L2 bar of the north star
+TOL = 1e-5  # relative L2 parity tolerance for floating-point results
 
 DEV = "cuda"
 
