@@ -1,4 +1,4 @@
-"""CPU tests that pin the oracle (the checker of every GPU parity test).
+"""CPU tests that pin the oracle (what every GPU parity test compares against).
 
 What pins what:
   * golden fixtures written by the reference's own compiled CPU path
