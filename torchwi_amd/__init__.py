@@ -12,5 +12,6 @@ the reference's per-process shot loop), with wavefield checkpointing.
 """
 from torchwi_amd.operator.TimeOperator import Time2d, Time2dForward  # noqa: F401
 from torchwi_amd.batch import Time2dBatch, Time2dWindows  # noqa: F401
+from torchwi_amd.inversion import TimeInversion  # noqa: F401
 
-__all__ = ["Time2d", "Time2dForward", "Time2dBatch", "Time2dWindows"]
+__all__ = ["Time2d", "Time2dForward", "Time2dBatch", "Time2dWindows", "TimeInversion"]
