@@ -69,21 +69,34 @@ def parse():
 
 
 def copy_probe(mb, reps=10):
-    """Achieved GB/s (read + write bytes) of a plain device copy of mb MiB."""
+    """Achieved GB/s (read + write bytes) of a device copy of mb MiB: torch's
+    copy, and the library's streaming copy (tw_stream_copy: one 16-byte load
+    and store per thread), the achievable HBM rate of this device."""
     import torch
+    from torchwi_amd import _lib
     n = mb * (1 << 20) // 4
     a = torch.rand(n, device="cuda:0")
     b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    return {"bytes_per_copy": 2 * n * 4, "avg_ms": round(ms, 4), "GBs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}
+    L = _lib.lib()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms = timed(lambda: b.copy_(a))
+    stream = torch.cuda.current_stream().cuda_stream
+    ms_s = timed(lambda: _lib.check(L.tw_stream_copy(b.data_ptr(), a.data_ptr(), 4 * n, stream), "tw_stream_copy"))
+    assert torch.equal(a, b)
+    gbs = lambda t: round(2 * n * 4 / (t * 1e-3) / 1e9, 1)  # noqa: E731
+    return {"bytes_per_copy": 2 * n * 4, "avg_ms": round(ms, 4), "GBs": gbs(ms),
+            "stream_copy_avg_ms": round(ms_s, 4), "stream_copy_GBs": gbs(ms_s)}
 
 
 def cpu_baseline():
@@ -301,6 +314,12 @@ def main():
                            "achieved": ach,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                            "traffic": tr, "bytes_per_cell": b, "cells_per_launch": cells_launch}
+        if tr and probe:
+            # measured HBM bytes of the launch over its duration, against the
+            # streaming copy of the same device
+            real = tr["bytes_per_cell"] * cells_launch / (per[dom]["avg_us"] * 1e-6) / 1e9
+            out["roofline"]["traffic_GBs"] = round(real, 1)
+            out["roofline"]["frac_of_stream_copy"] = round(real / probe["stream_copy_GBs"], 4)
     if args.compare:
         if kstate == 0:
             rc = run_batched(c, kck, 1, 0, False, args.graph_steps, dev, tb, args.tile_rows, args.tile_cols, gg)

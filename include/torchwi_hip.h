@@ -187,6 +187,14 @@ int tw_plan_set_timing(void *plan, int enable);
  * TW_NKINDS); waits for the recorded events.  reset != 0 clears the totals. */
 int tw_plan_timing(void *plan, int reset, double *ms, long long *count);
 
+/* ------------------------------------------------------------------------
+ * Measurement aid.  No counterpart in the reference: a plain streaming copy
+ * of `bytes` bytes (a multiple of 16; both pointers 16-byte aligned, device
+ * memory, not overlapping), one 16-byte load and store per thread, used by
+ * bench.py as the achievable HBM rate of the same device.
+ * ---------------------------------------------------------------------- */
+int tw_stream_copy(void *dst, const void *src, size_t bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

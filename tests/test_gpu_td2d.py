@@ -478,3 +478,26 @@ def test_auto_time_block_large_plan_bitexact():
     assert torch.equal(res[0][1], res[1][1])
     for a, b in zip(res[0][2], res[1][2]):
         assert torch.equal(a, b)
+
+
+def test_stream_copy():
+    """The measurement copy (tw_stream_copy) copies exactly, including a
+    size that is not a multiple of the workgroup, and refuses sizes and
+    pointers that are not 16-byte multiples and overlapping buffers."""
+    from torchwi_amd import _lib
+    L = _lib.lib()
+    s = torch.cuda.current_stream().cuda_stream
+    for n in (1 << 20, 4004):
+        a = torch.rand(n, device=DEV)
+        b = torch.zeros_like(a)
+        assert L.tw_stream_copy(b.data_ptr(), a.data_ptr(), 4 * n, s) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(a, b)
+    a = torch.rand(64, device=DEV)
+    b = torch.zeros_like(a)
+    assert L.tw_stream_copy(b.data_ptr(), a.data_ptr(), 12, s) == -1
+    assert L.tw_stream_copy(b.data_ptr() + 4, a.data_ptr(), 16, s) == -1
+    assert L.tw_stream_copy(a.data_ptr() + 16, a.data_ptr(), 64, s) == -1
+    assert "overlap" in _lib.last_error()
+    torch.cuda.synchronize()
+    assert not torch.any(b != 0)

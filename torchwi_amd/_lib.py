@@ -36,6 +36,7 @@ SIGNATURES = {
     "tw_plan_destroy": (None, [_vp]),
     "tw_plan_set_timing": (_ci, [_vp, _ci]),
     "tw_plan_timing": (_ci, [_vp, _ci, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]),
+    "tw_stream_copy": (_ci, [_vp, _vp, ctypes.c_size_t, _vp]),
 }
 
 # Step-kernel kinds of tw_plan_timing (TW_KIND_* in the header).

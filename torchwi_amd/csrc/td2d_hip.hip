@@ -999,6 +999,15 @@ int launch_batch(Plan& p, StepArgs a, hipStream_t s, bool timed = true)
     return TW_OK;
 }
 
+// Streaming copy, one 16-byte element per thread (the achievable-bandwidth
+// reference of the measurements).
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b,
+                                                     long long n4)
+{
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    if (i < n4) gst4((float*)(b + i), gld4((const float*)(a + i)));
+}
+
 int set_ctl(const Plan& p, int idx, int op, int v, hipStream_t s)
 {
     hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, s, p.d_ctl + idx, op, v);
@@ -1180,6 +1189,24 @@ extern "C" {
 const char* tw_last_error(void) { return g_err.c_str(); }
 
 const char* tw_version(void) { return "torchwi_amd td2d 0.1 gfx950"; }
+
+int tw_stream_copy(void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (!dst || !src) return fail(TW_EINVAL, "tw_stream_copy: NULL pointer");
+    if (bytes % 16 != 0 || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        return fail(TW_EINVAL, "tw_stream_copy: size and pointers must be multiples of 16 bytes");
+    const char* d = (const char*)dst;
+    const char* a = (const char*)src;
+    if (d < a + bytes && a < d + bytes) return fail(TW_EINVAL, "tw_stream_copy: buffers overlap");
+    const long long n4 = (long long)(bytes / 16);
+    if (n4 == 0) return TW_OK;
+    const long long blocks = (n4 + 255) / 256;
+    if (blocks > 0x7fffffffLL) return fail(TW_EINVAL, "tw_stream_copy: %zu bytes is too large", bytes);
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       (const float4*)src, (float4*)dst, n4);
+    TW_HIP(hipGetLastError());
+    return TW_OK;
+}
 
 int tw_td2d_forward(float* frd, float* virt, float* u1, float* u2, float* u3, const float* vpad,
                     const float* w, int order, int dimx, int dimy, int nt, float h, float dt,
