@@ -501,3 +501,39 @@ def test_stream_copy():
     assert "overlap" in _lib.last_error()
     torch.cuda.synchronize()
     assert not torch.any(b != 0)
+
+
+def test_backward_after_a_later_forward_raises():
+    """The saved wavefield storage of a forward is the module's own buffer
+    (as in the reference); a backward whose forward is no longer the
+    module's latest raises instead of returning a wrong gradient, and the
+    latest forward's backward still works and equals the oracle."""
+    import torchwi_amd as torchwi
+    order = 8
+    c = small_case(order, nx=40, ny=24, nt=60, seed=31)
+    w = torch.from_numpy(c["w"])
+    pos = [torch.tensor([c["sx"]]), torch.tensor([c["sy"]]), torch.tensor([c["ry"]])]
+    m = torchwi.Time2d(c["nx"], c["ny"], c["h"], w, c["dt"], order, DEV)
+    v = torch.nn.Parameter(torch.from_numpy(c["vel"]).to(DEV))
+    f1 = m(v, *pos)
+    l1 = f1.pow(2).sum()
+    f2 = m(v, *pos)
+    l2 = f2.pow(2).sum()
+    with pytest.raises(RuntimeError, match="another forward"):
+        l1.backward()
+    l2.backward()
+    vpad = O.pad_vel(c["vel"], order)
+    args = (order, c["dimx"], c["dimy"], c["h"], c["dt"])
+    frd_o, virt_o = O.forward(vpad, c["w"], *args, c["sx"], c["sy"], c["ry"])
+    g_o = O.crop_grad(O.backward(virt_o, vpad, 2 * frd_o, *args, c["ry"]), order, c["nx"], c["ny"])
+    assert np.array_equal(v.grad.cpu().numpy(), g_o)
+    # the batched module keeps the same rule
+    mb = torchwi.Time2dBatch(c["nx"], c["ny"], c["h"], w, c["dt"], order, DEV, ckpt_every=0)
+    vb = torch.nn.Parameter(torch.from_numpy(c["vel"]).to(DEV))
+    s3 = [c["sx"]], [c["sy"]], [c["ry"]]
+    lb1 = mb(vb, *s3).pow(2).sum()
+    lb2 = mb(vb, *s3).pow(2).sum()
+    with pytest.raises(RuntimeError, match="another forward"):
+        lb1.backward()
+    lb2.backward()
+    assert np.array_equal(vb.grad.cpu().numpy(), g_o)

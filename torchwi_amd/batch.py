@@ -67,12 +67,17 @@ class _BatchOperator(torch.autograd.Function):
         sx, sy, ry, m = args
         p = m.prepare(vel, sx, sy, ry)
         p.forward()
-        ctx.m, ctx.p = m, p
+        m._fwd_generation = getattr(m, "_fwd_generation", 0) + 1
+        ctx.m, ctx.p, ctx.generation = m, p, m._fwd_generation
         return p.frd_all()
 
     @staticmethod
     def backward(ctx, grad_output):
         m, p = ctx.m, ctx.p
+        if m._fwd_generation != ctx.generation:
+            # the plan's stored virtual sources / checkpoints now belong to a later forward
+            raise RuntimeError("the module ran another forward after this one: its saved wavefield storage "
+                               "was overwritten; call backward before the module's next forward")
         p.resid_all().copy_(grad_output)
         p.backward()
         return m.reduce_grad(p), None
@@ -169,6 +174,7 @@ class Time2dBatch(torch.nn.Module):
         Returns (loss, grad (nx, ny))."""
         p = self.prepare(vel, sx, sy, ry)
         p.forward()
+        self._fwd_generation = getattr(self, "_fwd_generation", 0) + 1
         r = p.resid_all()
         torch.sub(p.frd_all(), d_obs, out=r)
         rv = r.view(-1)
@@ -180,6 +186,7 @@ class Time2dBatch(torch.nn.Module):
     def modeling(self, vel, sx, sy, ry):
         p = self.prepare(vel, sx, sy, ry)
         p.forward()
+        self._fwd_generation = getattr(self, "_fwd_generation", 0) + 1
         return p.frd_all()
 
 

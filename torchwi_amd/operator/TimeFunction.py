@@ -5,6 +5,13 @@ forward_only)`` returns ``Function.apply`` taking ``(vel, (sx, sy, ry, module))`
 and reading/writing the module's buffers.  The GPU backends are the gfx950
 modules in torchwi_amd.propagator.  There is no CPU path in this package: a
 CPU device raises NotImplementedError instead of falling back.
+
+As in the reference, the virtual sources (or EXA arrays) saved for backward
+are the module's own buffers (TimeFunction.py:128-130), so a second forward
+on the same module overwrites what the first one's backward needs.  The
+reference then silently returns a wrong gradient; here every forward stamps
+the module with a generation number and a backward whose forward is no
+longer the module's latest raises instead.
 """
 import torch
 
@@ -33,6 +40,17 @@ def _crop_grad(grad, m):
     """Padded-grid gradient (dimy*dimx) -> (nx_org, ny_org), TimeFunction.py:141-145."""
     g = grad.view(m.dimy, m.dimx)[m.ne:m.ne + m.ny_org, m.ne:m.ne + m.nx_org]
     return g.transpose(0, 1)
+
+
+def _stamp(ctx, m):
+    m._fwd_generation = getattr(m, "_fwd_generation", 0) + 1
+    ctx.generation = m._fwd_generation
+
+
+def _check_stamp(ctx, m):
+    if getattr(m, "_fwd_generation", 0) != ctx.generation:
+        raise RuntimeError("the module ran another forward after this one: its saved wavefield storage "
+                           "was overwritten; call backward before the module's next forward")
 
 
 def get_operator(device, prop="ext", exa=False, forward_only=False):
@@ -68,6 +86,7 @@ def get_operator(device, prop="ext", exa=False, forward_only=False):
                                     m.order, m.dimx, m.dimy, m.nt, m.h, m.dt,
                                     sx.item(), sy.item(), ry.item())
                 ctx.model = m
+                _stamp(ctx, m)
                 ctx.save_for_backward(m.exa, m.iexa, ry)
                 return m.frd.view(m.nt, m.nx)[:, :m.nx_org]
 
@@ -75,6 +94,7 @@ def get_operator(device, prop="ext", exa=False, forward_only=False):
             def backward(ctx, grad_output):
                 exa, iexa, ry = ctx.saved_tensors
                 m = ctx.model
+                _check_stamp(ctx, m)
                 grad = _propagator.backward(exa, iexa, m.u1, m.u2, m.u3, m.vpad, grad_output,
                                             m.order, m.dimx, m.dimy, m.nt, m.h, m.dt, ry.item())
                 return _crop_grad(grad, m), None
@@ -90,6 +110,7 @@ def get_operator(device, prop="ext", exa=False, forward_only=False):
                                 m.order, m.dimx, m.dimy, m.nt, m.h, m.dt,
                                 sx.item(), sy.item(), ry.item())
             ctx.model = m
+            _stamp(ctx, m)
             ctx.save_for_backward(m.virt, ry)
             return m.frd.view(m.nt, m.nx)[:, :m.nx_org]
 
@@ -97,6 +118,7 @@ def get_operator(device, prop="ext", exa=False, forward_only=False):
         def backward(ctx, grad_output):
             virt, ry = ctx.saved_tensors
             m = ctx.model
+            _check_stamp(ctx, m)
             grad = _propagator.backward(virt, m.u1, m.u2, m.u3, m.vpad, grad_output,
                                         m.order, m.dimx, m.dimy, m.nt, m.h, m.dt, ry.item())
             return _crop_grad(grad, m), None
