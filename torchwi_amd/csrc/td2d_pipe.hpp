@@ -9,7 +9,7 @@
 // rows, one row per wave per iteration, with one wave per time level:
 //
 //   wave 0 (loader) streams the two input fields (levels -1 and 0) from HBM
-//          into LDS, PD rows ahead;
+//          into LDS, PD rows ahead (through register queues);
 //   wave k (level k) at iteration i computes level k of row r = O + i - 5k.
 //
 // Level k lags level k-1 by five rows (the stencil radius, 4, plus one), so
@@ -87,12 +87,6 @@ __device__ __forceinline__ float4 right4(float4 v)
 {
     return make_float4(dpp_from_right(v.x), dpp_from_right(v.y), dpp_from_right(v.z), dpp_from_right(v.w));
 }
-// Value at offset o in [-4, 7] of a row given as the float4s at x-4, x, x+4.
-__device__ __forceinline__ float at3(const float4& a, const float4& b, const float4& c, int o)
-{
-    return o < 0 ? f4at(a, o + 4) : (o < 4 ? f4at(b, o) : f4at(c, o - 4));
-}
-
 template <int NE, unsigned FLAGS, int T>
 __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restrict__ descs,
                                                        const int2* __restrict__ tiles, TArgs a)
@@ -177,10 +171,9 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
     // Every wave issues every stream's load at every iteration (from cell 0
     // when the value will not be used), so the in-order count of outstanding
     // vector-memory operations is the same on every path and the wait for one
-    // queue slot never waits for the slots behind it.
-    // The adjoint keeps three queues live (model, virtual source, and the
-    // gradient rows wave 1 starts from) and prefetches one iteration ahead;
-    // the other variants two.
+    // queue slot never waits for the slots behind it.  Every variant
+    // prefetches PD iterations ahead (the adjoint keeps three queues live:
+    // model, virtual source, and the gradient rows wave 1 starts from).
     constexpr int pd = PD;
     float4 qA[PQ], qB[PQ], qC[PQ];
     auto issue = [&](int slot_i, float4& A, float4& B, float4& C) {
