@@ -1,0 +1,83 @@
+"""The library's own source on the CPU: td2d_hip.hip (plan code, graph
+capture and replay, and the one-step kernels) is compiled for the host
+against a stand-in HIP runtime (tests/cpu_emu/hip/hip_runtime.h: one host
+thread per GPU thread, a barrier for __syncthreads) with AddressSanitizer,
+and run on small batched plans.  Every access of the kernels and of the
+host code is bounds-checked, and the gathers and per-shot gradients must
+equal the oracle's bit for bit (the host's IEEE float and double arithmetic
+is what the kernels are written to reproduce).  The temporally blocked
+kernel uses wavefront shifts that this stand-in does not provide and is not
+run here."""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "torchwi_amd", "csrc")
+EMU = os.path.join(ROOT, "tests", "cpu_emu")
+CXX = "/opt/rocm/llvm/bin/clang++"
+
+
+@pytest.fixture(scope="session")
+def driver(tmp_path_factory):
+    if not os.path.exists(CXX):
+        pytest.skip("no host clang++")
+    d = str(tmp_path_factory.mktemp("cpu_emu"))
+    shutil.copy(os.path.join(CSRC, "td2d_hip.hip"), os.path.join(d, "td2d_hip.cpp"))
+    shutil.copy(os.path.join(CSRC, "td2d_arith.hpp"), d)
+    vec = open(os.path.join(CSRC, "td2d_vec.hpp")).read().replace("__attribute__((address_space(1)))", "")
+    open(os.path.join(d, "td2d_vec.hpp"), "w").write(vec)
+    pipe = open(os.path.join(CSRC, "td2d_pipe.hpp")).read()
+    targs = re.search(r"struct TArgs \{.*?\n\};", pipe, flags=re.S).group(0)
+    open(os.path.join(d, "td2d_pipe.hpp"), "w").write(
+        "#pragma once\nnamespace tw {\n%s\nconstexpr int PLANES = 64;\n"
+        "template <int NE, unsigned FLAGS, int T> void k_pipe(const ShotDesc*, const int2*, TArgs) {}\n}\n" % targs)
+    flags = [CXX, "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-ffp-contract=off",
+             "-pthread", "-w", "-I", d, "-I", EMU, "-I", os.path.join(ROOT, "include")]
+    subprocess.run(flags + ["-c", "-o", os.path.join(d, "lib.o"), os.path.join(d, "td2d_hip.cpp")], check=True)
+    exe = os.path.join(d, "driver")
+    subprocess.run(flags + ["-o", exe, os.path.join(EMU, "driver.cpp"), os.path.join(d, "lib.o")], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("order,nx,ny,nt,S,K,graph", [
+    (8, 20, 12, 10, 2, 0, 4), (4, 37, 15, 9, 3, 4, 0), (2, 70, 18, 8, 2, 3, 3)])
+def test_kernel_source_on_host_threads_matches_oracle(driver, tmp_path, order, nx, ny, nt, S, K, graph):
+    rng = np.random.default_rng(order * 7 + nx)
+    h, dt = 10.0, 1e-3
+    vel = (1500.0 + 2500.0 * rng.random((nx, ny))).astype(np.float32)
+    w = rng.standard_normal(nt).astype(np.float32)
+    dimx, dimy = nx + order, ny + order
+    ld = (dimx + 31) // 32 * 32
+    vpad = O.pad_vel(vel, order).reshape(dimy, dimx)
+    vld = np.zeros((dimy, ld), np.float32)
+    vld[:, :dimx] = vpad
+    pos = np.array([[(rng.integers(0, nx) + 0.5) * h, (rng.integers(0, ny) + 0.5) * h,
+                     (rng.integers(0, ny) + 0.5) * h] for _ in range(S)], np.float32)
+    resid = rng.standard_normal((S, nt, nx)).astype(np.float32)
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    with open(fin, "wb") as f:
+        np.array([order, nx, ny, nt, S, K, graph], np.int32).tofile(f)
+        np.array([h, dt], np.float32).tofile(f)
+        for a in (vld, w, pos, resid):
+            a.tofile(f)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([driver, fin, fout], capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = np.fromfile(fout, np.float32)
+    frd = out[:S * nt * nx].reshape(S, nt, nx)
+    grad = out[S * nt * nx:S * nt * nx + S * dimy * ld].reshape(S, dimy, ld)
+    assert out[-1:].view(np.int32)[0] > 0
+    for s in range(S):
+        sx, sy, ry = (float(v) for v in pos[s])
+        frd_o, virt_o = O.forward(vpad, w, order, dimx, dimy, h, dt, sx, sy, ry)
+        assert np.array_equal(frd[s], frd_o), f"shot {s} gather"
+        g_o = O.backward(virt_o, vpad, resid[s], order, dimx, dimy, h, dt, ry).reshape(dimy, dimx)
+        assert np.array_equal(grad[s][:, :dimx], g_o), f"shot {s} gradient"
+        assert not np.any(grad[s][:, dimx:] != 0), f"shot {s} row padding"
