@@ -1,11 +1,12 @@
 // Host-thread run of the library's batched plan (td2d_hip.hip compiled for
 // the CPU against the stand-in runtime in hip/hip_runtime.h), for the CPU
-// test suite: the one-step kernels' source executes under AddressSanitizer,
-// and its gathers and gradients are compared with the oracle.
+// test suite: the kernels' source executes under AddressSanitizer, and the
+// gathers and gradients are compared with the oracle.
 //
 //   driver IN OUT
 //
-// IN (little-endian): int32 order, nx, ny, nt, S, K, graph; float32 h, dt;
+// IN (little-endian): int32 order, nx, ny, nt, S, K, graph, time_block,
+// tile_rows; float32 h, dt;
 // float32 vel[dimy*ld] (padded model, row stride ld = dimx rounded up to 32);
 // float32 w[nt]; float32 pos[S*3] (sx, sy, ry); float32 resid[S*nt*nx].
 // OUT: float32 frd[S*nt*nx], float32 grad[S*dimy*ld], int32 launches.
@@ -18,6 +19,8 @@
 
 thread_local uint3_ threadIdx, blockIdx, blockDim, gridDim;
 pthread_barrier_t g_bar;
+pthread_barrier_t g_wbar[EMU_MAX_WAVES];
+int g_dpp[EMU_MAX_WAVES][64];
 std::vector<std::function<void()>>* g_capture = nullptr;
 long long g_launches = 0;
 
@@ -35,11 +38,12 @@ int main(int argc, char** argv)
     if (argc != 3) return 2;
     FILE* f = fopen(argv[1], "rb");
     if (!f) return 2;
-    int hdr[7];
+    int hdr[9];
     float hd[2];
-    rd(f, hdr, 7);
+    rd(f, hdr, 9);
     rd(f, hd, 2);
-    const int order = hdr[0], nx = hdr[1], ny = hdr[2], nt = hdr[3], S = hdr[4], K = hdr[5], graph = hdr[6];
+    const int order = hdr[0], nx = hdr[1], ny = hdr[2], nt = hdr[3], S = hdr[4], K = hdr[5], graph = hdr[6],
+              tb = hdr[7], tile_rows = hdr[8];
     const int dimx = nx + order, dimy = ny + order, ld = (dimx + 31) / 32 * 32;
     const long long nf1 = (long long)dimy * ld, nf = nf1 * S;
     const int RING = 4, slots = K ? K : nt, nseg = K ? (nt + K - 1) / K : 0;
@@ -60,7 +64,8 @@ int main(int argc, char** argv)
     spec.dt = hd[1];
     spec.ckpt_every = K;
     spec.graph_steps = graph;
-    spec.time_block = 1;
+    spec.time_block = tb;
+    spec.tile_rows = tile_rows;
     std::vector<TwShotSpec> sh(S);
     for (int s = 0; s < S; ++s) {
         TwShotSpec& c = sh[s];

@@ -1,7 +1,8 @@
 // CPU stand-in for the HIP runtime: runs the library's host code and its
-// one-step kernels on host threads (one pthread per GPU thread, a pthread
-// barrier for __syncthreads), synchronously, with graph capture recorded as
-// a list of launches.  For AddressSanitizer runs only.
+// kernels on host threads (one pthread per GPU thread, a pthread barrier for
+// __syncthreads, per-wave barriers for wavefront operations), synchronously,
+// with graph capture recorded as a list of launches.  For AddressSanitizer
+// runs only.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -25,6 +26,28 @@ struct uint3_ { unsigned x, y, z; };
 extern thread_local uint3_ threadIdx, blockIdx, blockDim, gridDim;
 extern pthread_barrier_t g_bar;
 inline void __syncthreads() { pthread_barrier_wait(&g_bar); }
+
+// Wavefront operations.  The 64 lanes of a wave run as 64 host threads, so a
+// wave barrier is a barrier of those threads, and a DPP lane shift passes
+// values through a per-wave buffer between two such barriers (every lane of
+// the wave must reach it, as every lane of a real wave executes it).
+constexpr int EMU_MAX_WAVES = 16;
+extern pthread_barrier_t g_wbar[EMU_MAX_WAVES];
+extern int g_dpp[EMU_MAX_WAVES][64];
+inline void __builtin_amdgcn_wave_barrier() { pthread_barrier_wait(&g_wbar[threadIdx.x >> 6]); }
+inline int __builtin_amdgcn_readfirstlane(int x) { return x; }   // callers pass wave-uniform values
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int, int, bool)
+{
+    const int w = (int)(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
+    g_dpp[w][l] = src;
+    pthread_barrier_wait(&g_wbar[w]);
+    // 0x138 wave_shr:1 (lane l reads lane l-1), 0x130 wave_shl:1 (lane l+1);
+    // a lane with no source keeps old
+    const int from = ctrl == 0x138 ? l - 1 : (ctrl == 0x130 ? l + 1 : -1);
+    const int v = (from >= 0 && from < 64) ? g_dpp[w][from] : old;
+    pthread_barrier_wait(&g_wbar[w]);
+    return v;
+}
 struct dim3 { unsigned x, y, z; dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {} };
 typedef int hipError_t;
 enum { hipSuccess = 0 };
@@ -44,6 +67,9 @@ inline void emu_run(dim3 g, dim3 b, std::function<void()> k)
     pthread_barrier_t next;
     pthread_barrier_init(&g_bar, nullptr, b.x);
     pthread_barrier_init(&next, nullptr, b.x);
+    const unsigned nw = (b.x + 63) / 64;
+    for (unsigned w = 0; w < nw && w < (unsigned)EMU_MAX_WAVES; ++w)
+        pthread_barrier_init(&g_wbar[w], nullptr, b.x - 64 * w < 64 ? b.x - 64 * w : 64);
     std::vector<std::thread> th;
     for (unsigned t = 0; t < b.x; ++t)
         th.emplace_back([&, t] {
@@ -58,6 +84,7 @@ inline void emu_run(dim3 g, dim3 b, std::function<void()> k)
                 }
         });
     for (auto& x : th) x.join();
+    for (unsigned w = 0; w < nw && w < (unsigned)EMU_MAX_WAVES; ++w) pthread_barrier_destroy(&g_wbar[w]);
     pthread_barrier_destroy(&next);
     pthread_barrier_destroy(&g_bar);
     ++g_launches;

@@ -300,7 +300,9 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
                     // worker lane per Keys cell computes it (lanes 0..NE-1 the
                     // left cells, lanes 4..3+NE the right ones), and the owners
                     // read the results back.  LDS operations of one wave are
-                    // performed in order, so no barrier is needed.
+                    // performed in order, so no workgroup barrier is needed; the
+                    // wave barriers only keep the compiler from moving the LDS
+                    // accesses of the three phases across each other.
                     if (keys_left || keys_right) {
                         const float4 PL = left4(P), PR = right4(P);
                         float* const ks = (float*)&S[ksb];
@@ -315,6 +317,7 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
                             float4* o = (float4*)(ks + 32 * sR);
                             o[0] = PL; o[1] = P; o[2] = PR; o[3] = Lf; o[4] = C; o[5] = Rf; o[6] = M;
                         }
+                        __builtin_amdgcn_wave_barrier();
                         int wx = -1, ws = 0, wd = 0;
                         if (keys_left && lane < NE) {
                             wx = lane;
@@ -333,6 +336,7 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
                                                          o[16 + c + 2 * wd], o[24 + c], d.h2, d.dt2, d.hdt);
                             ks[96 + 4 * ws + c] = kv;
                         }
+                        __builtin_amdgcn_wave_barrier();
                         if (ownL) {
                             const float4 kr = *(const float4*)(ks + 96);
 #pragma unroll
