@@ -12,20 +12,20 @@
 //          into LDS, PD rows ahead (through register queues);
 //   wave k (level k) at iteration i computes level k of row r = O + i - 5k.
 //
-// Level k lags level k-1 by five rows (the stencil radius, 4, plus one), so
+// Level k lags level k-1 by LAG = 5 rows (the stencil radius, 4, plus one), so
 // every row it needs of level k-1 was produced at an earlier iteration and one
 // barrier per iteration orders the whole pipeline.  Each level wave keeps the
 // nine rows r-4..r+4 of level k-1 in a ten-slot register window (the row loop
 // is unrolled ten times, so every window slot is a fixed register); the x
 // neighbours of the centre row come from the adjacent lanes through DPP
 // wavefront shifts.  LDS carries only what moves between levels: the newest
-// row of each level (two slots), the rows of level k-2 that the next level
-// uses as its u_prev (read five iterations later: six slots; the row above,
-// which a bottom Keys cell also needs, is the u_prev the wave read one
-// iteration earlier and kept), and
-// in the adjoint the gradient sum after each level but the last (six slots;
-// wave 1 loads the stored gradient itself), so the
-// gradient is read and written once per launch and accumulated in time order.
+// row of each level (NCUR = 2 slots), the rows of level k-2 that the next
+// level uses as its u_prev (read LAG iterations later: NPRV = 6 slots; the row
+// above, which a bottom Keys cell also needs, is the u_prev the wave read one
+// iteration earlier and kept), and in the adjoint the gradient sum after each
+// level but the last (NGRD = 6 slots; wave 1 loads the stored gradient
+// itself), so the gradient is read and written once per launch and
+// accumulated in time order.
 //
 // The halo of a level is valid where its dependency cone stays inside the
 // loaded region; it shrinks by one float4 column and four rows per level on
@@ -51,14 +51,23 @@ constexpr int PLANES = 64;  // float4 columns per strip (one wavefront)
 constexpr int PW = 10;      // register window slots (the row loop's unroll)
 constexpr int PQ = 5;       // register prefetch slots (divides PW)
 constexpr int PD = 2;      // prefetch distance in iterations (< PQ)
-constexpr int NPRV = 6;     // LDS slots of level k-2 rows (read 5 iterations after the write)
-constexpr int NGRD = 6;     // LDS slots of gradient sums (read 5 iterations after the write)
+// Schedule: a workgroup barrier every BI iterations; level k lags level
+// k-1 by LAG = 4 + BI rows, so that each row a level reads was written at
+// least one barrier earlier, and the LDS rings are deep enough for the waves
+// of a workgroup to be up to BI - 1 iterations apart between barriers.
+// BI = 2 (a barrier every two rows) was measured no faster than BI = 1.
+constexpr int BI = 1;           // iterations per workgroup barrier
+constexpr int LAG = 4 + BI;     // rows between consecutive levels: stencil radius + BI
+constexpr int NCUR = LAG - 4 + BI;  // LDS slots of the newest rows of a level
+constexpr int NPRV = LAG + BI;  // LDS slots of level k-2 rows (read LAG iterations after the write)
+constexpr int NGRD = LAG + BI;  // LDS slots of gradient sums (read LAG iterations after the write)
 static_assert(NPRV == NGRD, "one slot counter serves both rings");
+static_assert(PW % BI == 0, "barriers fall on fixed iterations of the unrolled loop");
 
 template <int T, unsigned FLAGS>
 struct PLayout {
-    static constexpr int CUR = 0;                          // ring k: 2 rows of level k
-    static constexpr int PRV = CUR + T * 2 * PLANES;       // ring k: NPRV rows of level k-1
+    static constexpr int CUR = 0;                          // ring k: NCUR rows of level k
+    static constexpr int PRV = CUR + T * NCUR * PLANES;    // ring k: NPRV rows of level k-1
     static constexpr int GRD = PRV + T * NPRV * PLANES;    // ring k-1: NGRD rows of gradient sums after level k
     static constexpr int KS = GRD + ((FLAGS & F_GRAD) != 0 ? (T - 1) * NGRD * PLANES : 0);  // per level: Keys scratch
     static constexpr int SIZE = KS + T * 32;
@@ -177,7 +186,7 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
     constexpr int pd = PD;
     float4 qA[PQ], qB[PQ], qC[PQ];
     auto issue = [&](int slot_i, float4& A, float4& B, float4& C) {
-        const int row = O + slot_i - 5 * k;
+        const int row = O + slot_i - LAG * k;
         const bool rowA = row >= aLo && row < aHi, rowB = row >= bLo && row < bHi;
         A = gld4(pA + ((aLane && rowA) ? row * ld + gx : 0));
         const int ib = (useB && bLane && rowB) ? row * ld + gx : 0;
@@ -224,8 +233,8 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
     }
 
     // LDS ring bases of this wave: it reads ring k-1 and writes ring k
-    const int curR = LY::CUR + (k > 0 ? k - 1 : 0) * 2 * PLANES + lane;
-    const int curW = LY::CUR + k * 2 * PLANES + lane;
+    const int curR = LY::CUR + (k > 0 ? k - 1 : 0) * NCUR * PLANES + lane;
+    const int curW = LY::CUR + k * NCUR * PLANES + lane;
     const int prvR = LY::PRV + (k > 0 ? k - 1 : 0) * NPRV * PLANES + lane;
     const int prvW = LY::PRV + k * NPRV * PLANES + lane;
     const int grdR = LY::GRD + (k > 1 ? k - 2 : 0) * NGRD * PLANES + lane;   // sum after level k-1
@@ -240,8 +249,9 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
     for (int q = 0; q < pd; ++q) issue(q, qA[q], qB[q], qC[q]);
 
     const int ET = ey1 ? dimy : E - 4 * T;
-    const int nit = ET - O + 5 * T;
-    int c6 = 0;   // iteration mod 6 (NPRV = NGRD)
+    const int nit = ET - O + LAG * T;
+    int cP = 0;   // iteration mod NPRV (= NGRD)
+    int cC = 0;   // iteration mod NCUR
     float4 Pk = make_float4(0.f, 0.f, 0.f, 0.f);   // u_prev read this iteration (level waves)
 
     auto body = [&](int i, auto UC) {
@@ -249,23 +259,25 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
         issue(i + pd, qA[(U + pd) % PQ], qB[(U + pd) % PQ], qC[(U + pd) % PQ]);
         if (k == 0) {
             // loader: rows O + i of both input fields (and the gradient)
-            S[curW + (U % 2) * PLANES] = qB[U % PQ];
-            S[prvW + c6 * PLANES] = qA[U % PQ];
+            S[curW + cC * PLANES] = qB[U % PQ];
+            S[prvW + cP * PLANES] = qA[U % PQ];
         } else {
-            // append row r+4 of level k-1 (written by wave k-1 last iteration)
-            win[U] = S[curR + ((U + 1) % 2) * PLANES];
-            // u_prev of row r (level k-2, written by wave k-1 five iterations
+            // append row r+4 of level k-1 (written by wave k-1 LAG-4
+            // iterations ago)
+            const int cCr = cC + NCUR - (LAG - 4) >= NCUR ? cC - (LAG - 4) : cC + NCUR - (LAG - 4);
+            win[U] = S[curR + cCr * PLANES];
+            // u_prev of row r (level k-2, written by wave k-1 LAG iterations
             // ago); P1 keeps the one of row r-1
             const float4 P1 = Pk;
-            const int c6r = c6 + 1 >= NPRV ? c6 + 1 - NPRV : c6 + 1;   // (i-5) mod 6
-            Pk = S[prvR + c6r * PLANES];
-            // gradient sum after level k-1 (wave k-1, five iterations ago);
+            const int cPr = cP + NPRV - LAG >= NPRV ? cP - LAG : cP + NPRV - LAG;   // (i-LAG) mod NPRV
+            Pk = S[prvR + cPr * PLANES];
+            // gradient sum after level k-1 (wave k-1, LAG iterations ago);
             // read as one 16-byte LDS access on every iteration (wave 1 reads
             // ring 0 and discards it), not inside the selection below, where it
             // would become four conditional 4-byte reads
             float4 Gk = make_float4(0.f, 0.f, 0.f, 0.f);
-            if constexpr ((FLAGS & F_GRAD) != 0) Gk = S[grdR + c6r * PLANES];
-            const int r = O + i - 5 * k;
+            if constexpr ((FLAGS & F_GRAD) != 0) Gk = S[grdR + cPr * PLANES];
+            const int r = O + i - LAG * k;
             if (active && r >= Ok && r < Ek) {
                 const float4 C = win[(U + PW - 4) % PW];
                 const float4 P = Pk;
@@ -409,17 +421,18 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
                     if (k == a.nsteps) {
                         if (orow && lcore) gst4(d.grad + i0, g);
                     } else {
-                        S[grdW + c6 * PLANES] = g;
+                        S[grdW + cP * PLANES] = g;
                     }
                 }
                 if (writes_next) {
-                    S[curW + (U % 2) * PLANES] = un;
-                    S[prvW + c6 * PLANES] = C;
+                    S[curW + cC * PLANES] = un;
+                    S[prvW + cP * PLANES] = C;
                 }
             }
         }
-        c6 = c6 + 1 == NGRD ? 0 : c6 + 1;
-        __syncthreads();
+        cP = cP + 1 == NPRV ? 0 : cP + 1;
+        cC = cC + 1 == NCUR ? 0 : cC + 1;
+        if constexpr (U % BI == BI - 1) __syncthreads();
     };
 
     for (int base = 0; base < nit; base += PW) {
