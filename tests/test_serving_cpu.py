@@ -1,7 +1,7 @@
 """Host-side checks of the serving loop's admission arithmetic (no GPU)."""
 import torch
 
-from torchwi_amd.serving import Request, chunk_plan, io_bytes, make_stream
+from torchwi_amd.serving import Request, chunk_plan, chunk_time_block, io_bytes, make_stream
 from torchwi_amd.shots import default_ckpt_interval, plan_bytes
 
 
@@ -20,7 +20,7 @@ def test_gradient_uses_full_storage_when_it_fits_the_cap():
 
 def test_gradient_chunks_with_checkpoints_under_a_small_cap():
     r = _req("gradient", nt=900)
-    K = default_ckpt_interval(r.nt)
+    K = default_ckpt_interval(r.nt, 3)
     three = plan_bytes([(r.nx, r.ny, 8)], [dict(model=0, nrec=r.nx)] * 3, r.nt, K)
     per, k2, b = chunk_plan(r, three + 1)
     assert (per, k2, b) == (3, K, three)
@@ -47,3 +47,15 @@ def test_stream_generator_is_seeded():
     assert all(torch.equal(x.vel, y.vel) for x, y in zip(a, b))
     arr = [x.arrival for x in a]
     assert arr == sorted(arr) and arr[0] > 0
+
+
+def test_chunk_time_block_by_kind_storage_and_size():
+    from torchwi_amd.shots import AUTO_BLOCK_CELLS
+    big = _req("gradient", nx=3600, ny=1200, nt=6000, S=12)
+    cells = (big.nx + 8) * (big.ny + 8)
+    n = -(-AUTO_BLOCK_CELLS // cells)
+    assert chunk_time_block(big, n, 78) == 3 and chunk_time_block(big, n - 1, 78) == 1
+    assert chunk_time_block(big, n, 0) == 3            # full storage
+    assert chunk_time_block(big, n, 77) == 1           # not a multiple of the block
+    assert chunk_time_block(_req("forward", nx=3600, ny=1200, S=12), 12, 0) == 1
+    assert chunk_plan(big, 10 ** 10)[1] % 3 == 0

@@ -692,12 +692,16 @@ int pipe_slots(int order, int T)
 // Work list of the temporally blocked kernel: every shot's grid split into
 // column strips whose core plus halo fits one wavefront (64 float4 columns;
 // cores as even as float4 columns allow) and into row chunks.  chunk_rows > 0
-// sets the chunk height.  0 picks the height that minimises (rounds of
-// resident workgroups) x (iterations of one workgroup, chunk + 9T rows): a
-// workgroup's rows march in sequence, so fewer, taller chunks are better as
-// long as they fill the device once.  Dealt model by model, chunk- and
-// strip-major and shot-minor, so an XCD's band of workgroups shares model
-// rows.
+// sets the chunk height.  0 picks the height that minimises
+//   tiles * (chunk + 4T) + 0.45 * slots * (chunk + 9T),
+// the rows of every tile with a per-tile overhead plus the tail of the last
+// round of resident workgroups (slots), a cost fitted to MI355X timings of
+// checkpointed gradients (7 plans of 12-128 shots on grids of 392-2008 rows,
+// 27 chunk heights of 33-404 rows: 5% rms).  Several chunks per workgroup
+// slot keep the pipelines on a CU in different phases (fill, march, drain);
+// one round of tall chunks starts and ends them together and measured up to
+// 45% slower.  Dealt model by model, chunk- and strip-major and shot-minor,
+// so an XCD's band of workgroups shares model rows.
 int pipe_tiles(Plan& p, int chunk_rows)
 {
     if (chunk_rows < 0 || (chunk_rows > 0 && chunk_rows < 16))
@@ -721,7 +725,7 @@ int pipe_tiles(Plan& p, int chunk_rows)
             const int r = (maxdy + n - 1) / n;
             long long tiles = 0;
             for (const ShotDesc& d : p.host) tiles += (long long)nstrips(d.dimx) * ((d.dimy + r - 1) / r);
-            const long long cost = ((tiles + slots - 1) / slots) * (long long)(r + 9 * T);
+            const long long cost = 20 * tiles * (r + 4 * T) + 9LL * slots * (r + 9 * T);
             if (best < 0 || cost < best) {
                 best = cost;
                 rows = r;

@@ -26,7 +26,7 @@ from typing import List, Optional
 import numpy as np
 import torch
 
-from .shots import GRAPH_STEPS, ShotPlan, default_ckpt_interval, plan_bytes
+from .shots import GRAPH_STEPS, ShotPlan, auto_time_block, default_ckpt_interval, plan_bytes
 
 
 @dataclass
@@ -98,10 +98,21 @@ def chunk_plan(req, cap_bytes):
     if req.kind == "gradient" and nbytes(req.nshots, 0) <= cap_bytes:
         # full storage when the whole request fits under the cap: less traffic
         return req.nshots, 0, nbytes(req.nshots, 0)
-    K = 0 if req.kind == "forward" else default_ckpt_interval(req.nt)
+    # a multiple of the blocked kernel's 3 steps, so either step kernel can run the chunk
+    K = 0 if req.kind == "forward" else default_ckpt_interval(req.nt, 3)
     base, one = nbytes(0, K), nbytes(1, K) - nbytes(0, K)
     per = max(1, min(req.nshots, int((cap_bytes - base) // max(one, 1))))
     return per, K, nbytes(per, K)
+
+
+def chunk_time_block(req, nshots, K):
+    """Steps per launch for a chunk of nshots shots: gradients by plan size
+    (shots.auto_time_block), forward modelling one step per launch (measured
+    faster at every size on MI355X, e.g. 12 shots of 3608 x 1208: 355 vs 307
+    Gcell-updates/s).  Either gives the same bits."""
+    if req.kind != "gradient" or K % 3:
+        return 1
+    return auto_time_block(nshots * (req.nx + req.order) * (req.ny + req.order))
 
 
 def io_bytes(req):
@@ -206,7 +217,8 @@ class Server:
         with torch.cuda.stream(a.stream):
             a.plan = None
             p = ShotPlan(req.order, req.nt, req.h, req.dt, [(req.nx, req.ny)], shots, a.K, self.device,
-                         graph_steps=self.graph_steps, forward_only=req.kind == "forward")
+                         graph_steps=self.graph_steps, forward_only=req.kind == "forward",
+                         time_block=chunk_time_block(req, s1 - s0, a.K))
             p.models[0].load(a.vel)
             p.forward()
             if req.kind == "gradient":

@@ -21,14 +21,16 @@ ALIGN = 32  # row stride alignment in floats (128 B)
 GRAPH_STEPS = 100  # time steps per captured hipGraph block (0: direct launches)
 TIME_BLOCK = 1     # time steps per launch of the step kernel (2..4: temporal blocking)
 # Cells per time step (all shots of a plan) from which the temporally blocked
-# kernel with 3 steps per launch is the default.  Measured on MI355X with
-# checkpointed gradients: below, the one-step kernel is faster because the
-# three wavefields of all shots largely stay in the 256 MB Infinity Cache
-# (32 shots of 1032 x 392 = 13M cells: 193 vs 128 Gcell/s; 128 shots, 52M
-# cells: 153 vs 130); above, the blocked kernel moves fewer HBM bytes and wins
-# (256 shots, 104M cells: 194 vs 147; 64 shots of 3008 x 1008, 194M cells:
-# 236 vs 181; 256 uneven windows, ~200M cells: picked by the autotuner).
-AUTO_BLOCK_CELLS = 75_000_000
+# kernel with 3 steps per launch is the default: where the three wavefields of
+# all shots (12 B per cell) outgrow MI355X's 256 MB Infinity Cache.  Measured
+# with checkpointed gradients (Gcell-updates/s, one step per launch / blocked):
+# below, the one-step kernel streams from the cache and wins (32 shots of
+# 1032 x 392, 13M cells: 194 vs 145; 48 shots, 19M: 177 vs 154; 3 shots of
+# 3608 x 1208, 13M: 209 vs 159); above, the blocked kernel moves fewer bytes
+# and wins (8 shots of 3008 x 1008, 24M: 168 vs 183; 64 shots of 1032 x 392,
+# 26M: 155 vs 167; 12 shots of 3608 x 1208, 52M: 171 vs 211; 64 shots of
+# 3008 x 1008, 194M: 181 vs 258).
+AUTO_BLOCK_CELLS = 22_000_000
 
 
 def auto_time_block(cells_per_step):
