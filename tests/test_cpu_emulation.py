@@ -44,7 +44,7 @@ def driver(tmp_path_factory):
 @pytest.mark.parametrize("order,nx,ny,nt,S,K,graph,tb,rows", [
     (8, 20, 12, 10, 2, 0, 4, 1, 0), (4, 37, 15, 9, 3, 4, 0, 1, 0), (2, 70, 18, 8, 2, 3, 3, 1, 0),
     (8, 24, 14, 9, 2, 0, 0, 3, 0), (8, 250, 26, 6, 1, 0, 0, 3, 16), (4, 40, 22, 6, 2, 4, 0, 2, 0),
-    (2, 33, 21, 8, 2, 4, 0, 4, 0)])
+    (2, 33, 21, 8, 2, 4, 0, 4, 0), (4, 40, 70, 6, 2, 3, 0, 3, 0)])
 def test_kernel_source_on_host_threads_matches_oracle(driver, tmp_path, order, nx, ny, nt, S, K, graph, tb, rows):
     rng = np.random.default_rng(order * 7 + nx)
     h, dt = 10.0, 1e-3
