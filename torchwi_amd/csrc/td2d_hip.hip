@@ -633,10 +633,12 @@ struct Plan {
     int ggroup = 1;                   // shots per gradient group of the one-step adjoint (1: per shot)
     std::vector<int2> gtiles;         // (group leader, by << 16 | bx): work list of the group adjoint
     int gntiles = 0, gper_xcd = 0;
+    std::vector<int2> vtiles;         // blocked kernel: the same work list shot-major (adjoint)
     std::vector<ModelRef> models;
     ShotDesc* d_desc = nullptr;
     int2* d_tiles = nullptr;
     int2* d_gtiles = nullptr;
+    int2* d_vtiles = nullptr;
     ~Plan()
     {
         g_fwd.release();
@@ -701,7 +703,8 @@ int pipe_slots(int order, int T)
 // slot keep the pipelines on a CU in different phases (fill, march, drain);
 // one round of tall chunks starts and ends them together and measured up to
 // 45% slower.  Dealt model by model, chunk- and strip-major and shot-minor,
-// so an XCD's band of workgroups shares model rows.
+// so an XCD's band of workgroups shares model rows; a second copy of the
+// list, shot-major, serves the adjoint (launch_t_nft).
 int pipe_tiles(Plan& p, int chunk_rows)
 {
     if (chunk_rows < 0 || (chunk_rows > 0 && chunk_rows < 16))
@@ -757,6 +760,9 @@ int pipe_tiles(Plan& p, int chunk_rows)
         for (int c = 0; c < nch; ++c)
             for (int j = 0; j < ns; ++j)
                 for (int s : group) p.tiles.push_back(make_int2(s, (c << 16) | j));
+        for (int s : group)
+            for (int c = 0; c < nch; ++c)
+                for (int j = 0; j < ns; ++j) p.vtiles.push_back(make_int2(s, (c << 16) | j));
     }
     p.ntiles = (int)p.tiles.size();
     p.per_xcd = (p.ntiles + NXCD - 1) / NXCD;
@@ -919,7 +925,7 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
 size_t plan_bytes(const Plan& p)
 {
     return align256(sizeof(ShotDesc) * p.nshots) + align256(sizeof(int2) * p.ntiles) + 256 +
-           align256(sizeof(int2) * p.gntiles);
+           align256(sizeof(int2) * p.gntiles) + align256(sizeof(int2) * p.vtiles.size());
 }
 
 template <int NE, unsigned FLAGS, int CX4>
@@ -1087,11 +1093,19 @@ int replay(Plan& p, Graph& gr, int kind, int steps, hipStream_t s)
     return TW_OK;
 }
 
+// The adjoint takes the work list shot-major (each XCD's band of workgroups
+// on one shot's virtual sources, wavefields and gradient), the forward steps
+// chunk-major and shot-minor (the band shares model rows).  Measured on
+// MI355X, us per step, shot-major vs chunk-major: adjoint 602 vs 692 (64
+// shots of 3008 x 1008), 288 vs 308 (192 of 1032 x 392), 189 vs 192 (12 of
+// 3608 x 1208); forward 406 vs 390 and recomputation 558 vs 569 (64 of
+// 3008 x 1008), recomputation 177 vs 165 (12 of 3608 x 1208).
 template <int NE, unsigned FLAGS, int T>
 void launch_t_nft(const Plan& p, const TArgs& a, hipStream_t s)
 {
+    const int2* tl = (FLAGS & F_GRAD) != 0 ? p.d_vtiles : p.d_tiles;
     hipLaunchKernelGGL((k_pipe<NE, FLAGS, T>), dim3(p.per_xcd * NXCD), dim3(64 * (T + 1)), 0, s, p.d_desc,
-                       p.d_tiles, a);
+                       tl, a);
 }
 
 template <int NE, unsigned FLAGS>
@@ -1319,12 +1333,16 @@ int tw_plan_create(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, 
     p->d_ctl = (int*)((char*)workspace + align256(sizeof(ShotDesc) * p->nshots) +
                       align256(sizeof(int2) * p->ntiles));
     p->d_gtiles = (int2*)((char*)p->d_ctl + 256);
+    p->d_vtiles = (int2*)((char*)p->d_gtiles + align256(sizeof(int2) * p->gntiles));
     p->graph_steps = spec->graph_steps;
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = hipMemcpyAsync(p->d_desc, p->host.data(), sizeof(ShotDesc) * p->nshots,
                                   hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
         e = hipMemcpyAsync(p->d_tiles, p->tiles.data(), sizeof(int2) * p->ntiles,
+                           hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && !p->vtiles.empty())
+        e = hipMemcpyAsync(p->d_vtiles, p->vtiles.data(), sizeof(int2) * p->vtiles.size(),
                            hipMemcpyHostToDevice, s);
     if (e == hipSuccess && p->gntiles > 0)
         e = hipMemcpyAsync(p->d_gtiles, p->gtiles.data(), sizeof(int2) * p->gntiles,
