@@ -9,7 +9,10 @@ where it finishes in seconds):
   shot 0 through the single-shot drop-in module (one direct launch per
   step, its own kernel) gives the same gather and gradient bit for bit;
 * 4 shots on the 1000 x 3000 model, nt = 8000, checkpoints every 90 steps:
-  one step per launch and 3 steps per launch agree bit for bit.
+  one step per launch and 3 steps per launch agree bit for bit;
+* 256 shots on uneven windows of the 600 x 4000 model, nt = 6000, in one
+  plan of the temporally blocked kernel: two of the shots run alone with
+  one step per launch give the same gathers and gradients bit for bit.
 """
 import numpy as np
 import pytest
@@ -93,3 +96,34 @@ def test_large_model_full_size_blocked_equals_one_step():
     assert np.isfinite(one[0]) and one[0] > 0
     print("case 4 shots 1000x3000 nt=8000: checkpoints every 90, 3 steps per launch", flush=True)
     _same(one, _gradients(c, vel, d_obs, S, ckpt_every=90, time_block=3), "blocked")
+
+
+def test_uneven_windows_full_size_batched_equals_alone():
+    from torchwi_amd import Time2dWindows
+    from torchwi_amd import workloads as W
+    c = W.uneven_600x4000()
+    order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
+    wins = c["windows"]
+    w = torch.from_numpy(c["w"])
+    vel = torch.from_numpy(c["vel"]).to(DEV)
+    print("case 256 uneven windows of 600x4000 nt=6000: batched, 3 steps per launch", flush=True)
+    m = Time2dWindows(nx, ny, h, w, dt, order, wins, DEV, time_block=3)
+    assert m.plan.time_block == 3 and m.plan.K > 0
+    d_obs = [f.clone() for f in m.modeling(vel * 1.01)]
+    loss, g = m.misfit_gradient(vel, d_obs)
+    assert np.isfinite(float(loss)) and float(loss) > 0
+    K = m.plan.K
+    for s in (0, len(wins) // 2):
+        print(f"case uneven window {s} alone, one step per launch", flush=True)
+        frd_b = m.plan.frd_of(s).clone()
+        g_b = m.plan.grad_of(s).clone()
+        one = Time2dWindows(nx, ny, h, w, dt, order, [wins[s]], DEV, ckpt_every=K, time_block=1)
+        f1 = one.modeling(vel * 1.01)[0]
+        assert torch.equal(f1, d_obs[s]), f"window {s}: gathers"
+        one.misfit_gradient(vel, [d_obs[s]])
+        assert torch.equal(one.plan.grad_of(0), g_b), f"window {s}: gradient"
+        assert torch.equal(one.plan.frd_of(0), frd_b), f"window {s}: gathers at the gradient's model"
+        del one
+        torch.cuda.empty_cache()
+    del m
+    torch.cuda.empty_cache()
