@@ -178,24 +178,30 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
             useB = false;
         }
     }
-    // Every wave issues every stream's load at every iteration (from cell 0
-    // when the value will not be used), so the in-order count of outstanding
-    // vector-memory operations is the same on every path and the wait for one
-    // queue slot never waits for the slots behind it.  Every variant
-    // prefetches PD iterations ahead (the adjoint keeps three queues live:
-    // model, virtual source, and the gradient rows wave 1 starts from).
+    // Every wave of a role issues the same loads at every iteration (from
+    // cell 0 when a row is outside its range), so the in-order count of
+    // outstanding vector-memory operations is the same on every path and the
+    // wait for one queue slot never waits for the slots behind it.  In the
+    // forward steps the loader and the level waves run separate copies of the
+    // row loop (ROLE 0 and 2), each issuing only the streams it uses; the
+    // adjoint keeps one loop for all waves (ROLE 3: three queues live on every
+    // wave -- model, virtual source, and the gradient rows wave 1 starts
+    // from), which measured faster for it.  Every variant prefetches PD
+    // iterations ahead.
     constexpr int pd = PD;
     float4 qA[PQ], qB[PQ], qC[PQ];
-    auto issue = [&](int slot_i, float4& A, float4& B, float4& C) {
+    auto issue = [&](auto RC, int slot_i, float4& A, float4& B, float4& C) {
+        constexpr int ROLE = decltype(RC)::value;
         const int row = O + slot_i - LAG * k;
         const bool rowA = row >= aLo && row < aHi, rowB = row >= bLo && row < bHi;
         A = gld4(pA + ((aLane && rowA) ? row * ld + gx : 0));
-        const int ib = (useB && bLane && rowB) ? row * ld + gx : 0;
         if constexpr ((FLAGS & F_GRAD) != 0) {
+            const int ib = (useB && bLane && rowB) ? row * ld + gx : 0;
             B = k > 0 ? gld4_nt(pB + ib) : gld4(pB + ib);
             const bool cok = k == 1 && lcore && row >= Y0 && row < Y1;
             C = gld4(d.grad + (cok ? row * ld + gx : 0));
-        } else {
+        } else if constexpr (ROLE == 0 || (FLAGS & F_VIRT) != 0) {
+            const int ib = (useB && bLane && rowB) ? row * ld + gx : 0;
             B = gld4(pB + ib);
         }
     };
@@ -246,8 +252,6 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
     float4 win[PW];
 #pragma unroll
     for (int q = 0; q < PW; ++q) win[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-    for (int q = 0; q < pd; ++q) issue(q, qA[q], qB[q], qC[q]);
 
     const int ET = ey1 ? dimy : E - 4 * T;
     const int nit = ET - O + LAG * T;
@@ -255,10 +259,11 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
     int cC = 0;   // iteration mod NCUR
     float4 Pk = make_float4(0.f, 0.f, 0.f, 0.f);   // u_prev read this iteration (level waves)
 
-    auto body = [&](int i, auto UC) {
+    auto body = [&](auto RC, int i, auto UC) {
+        constexpr int ROLE = decltype(RC)::value;
         constexpr int U = decltype(UC)::value;
-        issue(i + pd, qA[(U + pd) % PQ], qB[(U + pd) % PQ], qC[(U + pd) % PQ]);
-        if (k == 0) {
+        issue(RC, i + pd, qA[(U + pd) % PQ], qB[(U + pd) % PQ], qC[(U + pd) % PQ]);
+        if (ROLE == 0 || (ROLE == 3 && k == 0)) {
             // loader: rows O + i of both input fields (and the gradient)
             S[curW + cC * PLANES] = qB[U % PQ];
             S[prvW + cP * PLANES] = qA[U % PQ];
@@ -436,17 +441,28 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
         if constexpr (U % BI == BI - 1) __syncthreads();
     };
 
-    for (int base = 0; base < nit; base += PW) {
-        body(base + 0, IC<0>{});
-        body(base + 1, IC<1>{});
-        body(base + 2, IC<2>{});
-        body(base + 3, IC<3>{});
-        body(base + 4, IC<4>{});
-        body(base + 5, IC<5>{});
-        body(base + 6, IC<6>{});
-        body(base + 7, IC<7>{});
-        body(base + 8, IC<8>{});
-        body(base + 9, IC<9>{});
+    auto march = [&](auto RC) {
+#pragma unroll
+        for (int q = 0; q < pd; ++q) issue(RC, q, qA[q], qB[q], qC[q]);
+        for (int base = 0; base < nit; base += PW) {
+            body(RC, base + 0, IC<0>{});
+            body(RC, base + 1, IC<1>{});
+            body(RC, base + 2, IC<2>{});
+            body(RC, base + 3, IC<3>{});
+            body(RC, base + 4, IC<4>{});
+            body(RC, base + 5, IC<5>{});
+            body(RC, base + 6, IC<6>{});
+            body(RC, base + 7, IC<7>{});
+            body(RC, base + 8, IC<8>{});
+            body(RC, base + 9, IC<9>{});
+        }
+    };
+    if constexpr ((FLAGS & F_GRAD) != 0) {
+        march(IC<3>{});
+    } else if (k == 0) {
+        march(IC<0>{});
+    } else {
+        march(IC<2>{});
     }
 }
 
