@@ -71,7 +71,12 @@ struct PLayout {
     static constexpr int PRV = CUR + T * NCUR * PLANES;    // ring k: NPRV rows of level k-1
     static constexpr int GRD = PRV + T * NPRV * PLANES;    // ring k-1: NGRD rows of gradient sums after level k
     static constexpr int KS = GRD + ((FLAGS & F_GRAD) != 0 ? (T - 1) * NGRD * PLANES : 0);  // per level: Keys scratch
-    static constexpr int SIZE = KS + T * 32;
+    static constexpr int SIZE0 = KS + T * 32;
+    // The adjoint's LDS is padded to 41.6 KB so that at most 3 workgroups (3
+    // waves per SIMD) share a CU: at 4 (its 126 VGPRs would allow them) it
+    // measured 4% slower at 64 shots of 3008 x 1008 and 10% slower at 192
+    // shots of 1032 x 392.
+    static constexpr int SIZE = ((FLAGS & F_GRAD) != 0 && SIZE0 < 2600) ? 2600 : SIZE0;
 };
 
 template <int N>
@@ -178,16 +183,15 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
             useB = false;
         }
     }
-    // Every wave of a role issues the same loads at every iteration (from
-    // cell 0 when a row is outside its range), so the in-order count of
-    // outstanding vector-memory operations is the same on every path and the
-    // wait for one queue slot never waits for the slots behind it.  In the
-    // forward steps the loader and the level waves run separate copies of the
-    // row loop (ROLE 0 and 2), each issuing only the streams it uses; the
-    // adjoint keeps one loop for all waves (ROLE 3: three queues live on every
-    // wave -- model, virtual source, and the gradient rows wave 1 starts
-    // from), which measured faster for it.  Every variant prefetches PD
-    // iterations ahead.
+    // Each wave role runs its own copy of the row loop (ROLE 0: the loader;
+    // 1: wave 1 of the adjoint, which starts from the stored gradient; 2: the
+    // other level waves) and issues exactly the loads it uses, the same ones
+    // at every iteration (from cell 0 when a row is outside its range), so
+    // the in-order count of outstanding vector-memory operations is the same
+    // on every path and the wait for one queue slot never waits for the slots
+    // behind it.  Loads are not free even when they hit: a placeholder load
+    // costs about as much as a real one (1-2% of the forward steps' time).
+    // Every variant prefetches PD iterations ahead.
     constexpr int pd = PD;
     float4 qA[PQ], qB[PQ], qC[PQ];
     auto issue = [&](auto RC, int slot_i, float4& A, float4& B, float4& C) {
@@ -197,9 +201,12 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
         A = gld4(pA + ((aLane && rowA) ? row * ld + gx : 0));
         if constexpr ((FLAGS & F_GRAD) != 0) {
             const int ib = (useB && bLane && rowB) ? row * ld + gx : 0;
-            B = k > 0 ? gld4_nt(pB + ib) : gld4(pB + ib);
-            const bool cok = k == 1 && lcore && row >= Y0 && row < Y1;
-            C = gld4(d.grad + (cok ? row * ld + gx : 0));
+            if constexpr (ROLE == 0) B = gld4(pB + ib);
+            else B = gld4_nt(pB + ib);
+            if constexpr (ROLE == 1) {
+                const bool cok = lcore && row >= Y0 && row < Y1;
+                C = gld4(d.grad + (cok ? row * ld + gx : 0));
+            }
         } else if constexpr (ROLE == 0 || (FLAGS & F_VIRT) != 0) {
             const int ib = (useB && bLane && rowB) ? row * ld + gx : 0;
             B = gld4(pB + ib);
@@ -263,7 +270,7 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
         constexpr int ROLE = decltype(RC)::value;
         constexpr int U = decltype(UC)::value;
         issue(RC, i + pd, qA[(U + pd) % PQ], qB[(U + pd) % PQ], qC[(U + pd) % PQ]);
-        if (ROLE == 0 || (ROLE == 3 && k == 0)) {
+        if constexpr (ROLE == 0) {
             // loader: rows O + i of both input fields (and the gradient)
             S[curW + cC * PLANES] = qB[U % PQ];
             S[prvW + cP * PLANES] = qA[U % PQ];
@@ -282,7 +289,7 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
             // ring 0 and discards it), not inside the selection below, where it
             // would become four conditional 4-byte reads
             float4 Gk = make_float4(0.f, 0.f, 0.f, 0.f);
-            if constexpr ((FLAGS & F_GRAD) != 0) Gk = S[grdR + cPr * PLANES];
+            if constexpr ((FLAGS & F_GRAD) != 0 && ROLE != 1) Gk = S[grdR + cPr * PLANES];
             const int r = O + i - LAG * k;
             if (active && r >= Ok && r < Ek) {
                 const float4 C = win[(U + PW - 4) % PW];
@@ -417,7 +424,9 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
                 if constexpr ((FLAGS & F_GRAD) != 0) {
                     // wave 1 starts from the stored gradient (its own queue),
                     // wave k > 1 from wave k-1's sum
-                    const float4 gin = k == 1 ? qC[U % PQ] : Gk;
+                    float4 gin;
+                    if constexpr (ROLE == 1) gin = qC[U % PQ];
+                    else gin = Gk;
                     const float4 v = qB[U % PQ];
                     float4 g;
                     g.x = gin.x + (-v.x) * un.x;
@@ -458,7 +467,9 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
         }
     };
     if constexpr ((FLAGS & F_GRAD) != 0) {
-        march(IC<3>{});
+        if (k == 0) march(IC<0>{});
+        else if (k == 1) march(IC<1>{});
+        else march(IC<2>{});
     } else if (k == 0) {
         march(IC<0>{});
     } else {
