@@ -50,7 +50,7 @@ struct TArgs {
 constexpr int PLANES = 64;  // float4 columns per strip (one wavefront)
 constexpr int PW = 10;      // register window slots (the row loop's unroll)
 constexpr int PQ = 5;       // register prefetch slots (divides PW)
-constexpr int PD = 2;      // prefetch distance in iterations (< PQ)
+constexpr int PD = 2;      // prefetch distance of the adjoint in iterations (< PQ); the forward steps use PD + 1
 // Schedule: a workgroup barrier every BI iterations; level k lags level
 // k-1 by LAG = 4 + BI rows, so that each row a level reads was written at
 // least one barrier earlier, and the LDS rings are deep enough for the waves
@@ -191,8 +191,11 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
     // on every path and the wait for one queue slot never waits for the slots
     // behind it.  Loads are not free even when they hit: a placeholder load
     // costs about as much as a real one (1-2% of the forward steps' time).
-    // Every variant prefetches PD iterations ahead.
-    constexpr int pd = PD;
+    // The adjoint prefetches PD iterations ahead, the forward steps PD + 1
+    // (their fewer registers leave room: 108-124 VGPRs, still 4 waves per
+    // SIMD; the forward 3-6% faster, the gradient at 64 shots of 3008 x 1008
+    // 0.6%).
+    constexpr int pd = (FLAGS & F_GRAD) != 0 ? PD : PD + 1;
     float4 qA[PQ], qB[PQ], qC[PQ];
     auto issue = [&](auto RC, int slot_i, float4& A, float4& B, float4& C) {
         constexpr int ROLE = decltype(RC)::value;
