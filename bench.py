@@ -1,21 +1,26 @@
-"""Benchmark: batched forward + adjoint gradient of the time-domain 2D
-acoustic propagator on one MI355X.
+"""Benchmark: checkpointed batched forward + adjoint gradient of the
+time-domain 2D acoustic propagator on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode auto|full|ckpt]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nt NT] [--compare 0|1] ...
 
-Workload (one "step" = one misfit gradient over every shot): the 384 x 1024
-model at h = 12.5 m, dt = 1 ms, nt = 4000, order 8, 32 ricker(8) shots at
-z = 2 with 1024 receivers, eight dipping layers 1500-4500 m/s with a +-5%
-correlated perturbation; observed data are modelled from a second seeded
-perturbation (untimed).  Inputs are resident in HBM before timing starts.
+Headline workload (one "step" = one misfit gradient over every shot): 64
+ricker(6) shots on the 1000 x 3000 dipping-layer model (padded 1008 x 3008,
+eight layers 1500-4500 m/s with a +-5% correlated perturbation), h = 10 m,
+dt = 0.8 ms, nt = 8000, order 8, 3000 receivers at z = 2.  Full
+virtual-source storage would need 97 GB per shot, so the gradient
+checkpoints the wavefield every 90 steps and recomputes each segment; the
+step kernel is the temporally blocked one (3 time steps per launch).
+Observed data are modelled from a 1%-faster model (untimed).  Inputs are
+resident in HBM before timing starts.
 
 Throughput is Gcell-updates/s with one cell-update = one padded-grid cell of
-one wavefield advanced one step for one shot; a gradient step is
+one wavefield advanced one step for one shot; a gradient is
 2 * shots * dimx * dimy * nt cell-updates (checkpoint recomputation is not
-credited).  Prints one JSON line.  Besides the headline (full storage, which
-fits in HBM here), the line carries the same workload checkpointed, the plain
-one-shot-per-launch form, and the checkpointed gradient of the 1000 x 3000
-model with 64 shots (nt = 8000), where checkpointing is required.
+credited).  Prints one JSON line.  Besides the headline the line carries the
+per-kernel roofline of the blocked kernel, the 384 x 1024 x 32-shot gradient
+(full storage and checkpointed, and the plain one-shot-per-launch form), the
+p50/p99 latency of a serving stream, and the reference CPU path on the
+host cores.
 """
 import argparse
 import json
@@ -30,11 +35,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0            # MI355X HBM3E, AMD spec
-BYTES_FWD = 16                   # u_prev, u_cur, model read; u_new write
-BYTES_FWD_VIRT = 20              # + virtual-source write
-BYTES_ADJ = 28                   # lambda x2 + model + virt + grad read; lambda + grad write
-BYTES_GRAD_STEP = BYTES_FWD + BYTES_ADJ   # credited per cell*step*shot (checkpointed form)
 METRIC = "Gcell-updates/s fwd+adj FP32 vs HBM roofline; gradient p50/p99 latency"
+# Per-unit algorithmic bytes of the one-step kernels (one cell-update):
+# forward u_prev, u_cur, model read + u_new write; + virtual-source write;
+# adjoint lambda x2, model, virt, grad read + lambda, grad write.
+STEP_BYTES = {"fwd": 16, "fwd_virt": 20, "recomp": 20, "adj": 28}
+BYTES_GRAD_STEP = 16 + 28        # credited per cell*step*shot of a gradient (22 B per cell-update)
+
+
+def pipe_bytes(T):
+    """Compulsory HBM bytes per cell of one launch of the temporally blocked
+    kernel advancing T steps (the intermediate levels stay on chip):
+      forward        two fields in, model m, two fields out        = 20
+      recomputation  + model c3, + T virtual sources written       = 24 + 4T
+      adjoint        + T virtual sources read, gradient in and out = 28 + 4T"""
+    return {"fwd": 20, "fwd_virt": 24 + 4 * T, "recomp": 24 + 4 * T, "adj": 28 + 4 * T}
 
 
 def parse():
@@ -42,29 +57,22 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="auto", choices=["auto", "full", "ckpt"])
-    ap.add_argument("--ckpt-every", type=int, default=0, help="K for --mode ckpt (0: ceil(sqrt(nt)))")
-    ap.add_argument("--nshots", type=int, default=32)
-    ap.add_argument("--nt", type=int, default=4000)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-timing", action="store_true", help="skip per-launch HIP-event timing")
-    ap.add_argument("--time-block", type=int, default=1,
-                    help="time steps per launch (2..4: temporally blocked kernel)")
+    ap.add_argument("--nshots", type=int, default=64)
+    ap.add_argument("--nt", type=int, default=8000)
+    ap.add_argument("--ckpt-every", type=int, default=0, help="K (0: ceil(sqrt(nt)) rounded to the time block)")
+    ap.add_argument("--time-block", type=int, default=0,
+                    help="time steps per launch (1: one step; 2..4: blocked kernel; 0: automatic by size)")
     ap.add_argument("--tile-rows", type=int, default=0,
-                    help="tile height (one step per launch: 32 or 64; blocked kernel: rows per chunk); 0 automatic")
-    ap.add_argument("--grad-group", type=int, default=1,
-                    help="shots per gradient group of the one-step adjoint (1: one gradient per shot; > 1 is refused by the library for now)")
-    ap.add_argument("--tile-cols", type=int, default=0,
-                    help="one-step kernel tile width: 64 or 128; 0 automatic")
-    ap.add_argument("--graph-steps", type=int, default=100,
-                    help="time steps per captured hipGraph block (0: one direct launch per step)")
+                    help="row-chunk height of the blocked kernel (0: automatic)")
+    ap.add_argument("--no-timing", action="store_true", help="skip per-launch HIP-event timing")
     ap.add_argument("--compare", type=int, default=1,
-                    help="also time the checkpointed form and the plain one-shot-per-launch form")
-    ap.add_argument("--no-scale", action="store_true",
-                    help="skip the checkpointed gradient of the 1000x3000 model with 64 shots (215 GB)")
+                    help="also time the 384 x 1024 x 32-shot gradient (full storage, checkpointed, plain form)")
+    ap.add_argument("--serving-requests", type=int, default=12,
+                    help="requests of the serving-stream leg (0: skip)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
-                    help="first time a device-to-device copy of MB megabytes (achievable-bandwidth "
-                         "reference and PMC byte-counter calibration)")
+                    help="first time a device copy of MB megabytes (achievable-bandwidth reference and "
+                         "PMC byte-counter calibration)")
     return ap.parse_args()
 
 
@@ -95,8 +103,11 @@ def copy_probe(mb, reps=10):
     ms_s = timed(lambda: _lib.check(L.tw_stream_copy(b.data_ptr(), a.data_ptr(), 4 * n, stream), "tw_stream_copy"))
     assert torch.equal(a, b)
     gbs = lambda t: round(2 * n * 4 / (t * 1e-3) / 1e9, 1)  # noqa: E731
-    return {"bytes_per_copy": 2 * n * 4, "avg_ms": round(ms, 4), "GBs": gbs(ms),
-            "stream_copy_avg_ms": round(ms_s, 4), "stream_copy_GBs": gbs(ms_s)}
+    out = {"bytes_per_copy": 2 * n * 4, "avg_ms": round(ms, 4), "GBs": gbs(ms),
+           "stream_copy_avg_ms": round(ms_s, 4), "stream_copy_GBs": gbs(ms_s)}
+    del a, b
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline():
@@ -127,52 +138,121 @@ def cpu_baseline():
         ts.append(time.perf_counter() - t0)
     t = statistics.median(ts)
     cells = 2.0 * dimx * dimy * nt
-    return {"value": round(cells / t / 1e9, 4), "unit": "Gcell-updates/s", "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
+    return {"value": round(cells / t / 1e9, 4), "unit": "Gcell-updates/s",
+            "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
             "kind": "reference" if impl == "ref" else "port",
             "sample": "100x200 model + 20-cell pad (248x128 padded, order 8), nt=1000, 1 shot, "
                       "forward with full storage + adjoint gradient; median of 5 runs (%.3f s each)" % t}
 
 
-def run_batched(c, K, steps, warmup, timing, graph_steps, dev, time_block=1, tile_rows=0, tile_cols=0,
-                grad_group=1):
-    """Time `steps` misfit gradients of the batched engine; returns stats."""
+def kernel_table(tm, cells, T):
+    """Per step-kernel kind: average launch duration and algorithmic GB/s
+    (the blocked kernel's compulsory bytes per launch, or the one-step
+    kernels' per-unit bytes) from the plan's HIP-event timing."""
+    per = {}
+    nb = pipe_bytes(T) if T > 1 else STEP_BYTES
+    for kind in ("fwd", "fwd_virt", "recomp", "adj"):
+        ms, steps = tm[kind]
+        if not steps:
+            continue
+        ms_step = ms / steps
+        b = nb[kind]
+        gbs = (b / T) * cells / (ms_step * 1e-3) / 1e9
+        per[kind] = {"avg_us_per_launch": round(ms_step * T * 1e3, 2), "steps": steps,
+                     "bytes_per_cell_per_launch": b, "achieved_GBs": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "share_of_time": ms}
+    tot = sum(v["share_of_time"] for v in per.values())
+    for v in per.values():
+        v["share_of_time"] = round(v["share_of_time"] / tot, 4)
+    return per
+
+
+def load_traffic(name):
+    """Calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE bytes per launch, kept in
+    profiles/ (see torchwi_amd.profiling)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def run_scale(args, dev):
+    """The headline: `steps` timed checkpointed misfit gradients of the
+    1000 x 3000 model after `warmup` untimed ones."""
     import torch
     from torchwi_amd import Time2dBatch
+    from torchwi_amd import workloads as W
+    from torchwi_amd.shots import auto_time_block, default_ckpt_interval
+    c = W.dipping_1000x3000(nshots=args.nshots, nt=args.nt)
     order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
     S = len(c["sx"])
-    model = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K,
-                        graph_steps=graph_steps, time_block=time_block, tile_rows=tile_rows,
-                        tile_cols=tile_cols, grad_group=grad_group)
+    dimx, dimy = nx + order, ny + order
+    tb = args.time_block or auto_time_block(S * dimx * dimy)
+    K = args.ckpt_every or default_ckpt_interval(nt, tb)
+    m = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K, time_block=tb,
+                    tile_rows=args.tile_rows)
     vel = torch.from_numpy(c["vel"]).to(dev)
-    rng = np.random.default_rng(7)
-    vtrue = torch.from_numpy(c["vel"] * (1.0 + 0.02 * rng.standard_normal(c["vel"].shape)).astype(np.float32)).to(dev)
-    d_obs = model.modeling(vtrue, c["sx"], c["sy"], c["ry"]).clone()
-    st = model.plan
-    mg = st.models[0]
+    d_obs = m.modeling(vel * 1.01, c["sx"], c["sy"], c["ry"]).clone()
     torch.cuda.synchronize()
-    for _ in range(warmup):
-        model.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
-    torch.cuda.synchronize()
+    for k in range(args.warmup):
+        m.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
+        torch.cuda.synchronize()
+        print("bench: warm-up gradient %d done" % (k + 1), file=sys.stderr, flush=True)
+    st = m.plan
+    timing = not args.no_timing
     if timing:
         st.set_timing(True)
         st.timing(reset=True)
     lat = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(args.steps):
         ts = time.perf_counter()
-        model.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
+        loss, g = m.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
         torch.cuda.synchronize()
-        lat.append((time.perf_counter() - ts) * 1e3)
+        lat.append(time.perf_counter() - ts)
+        print("bench: gradient %d: %.3f s" % (k + 1, lat[-1]), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    cells_step = 2.0 * S * mg.dimx * mg.dimy * nt
-    r = {"K": st.K, "S": S, "dimx": mg.dimx, "dimy": mg.dimy, "elapsed": elapsed, "lat": lat,
-         "value": cells_step * steps / elapsed / 1e9, "shots_per_s": S * steps / elapsed}
+    assert np.isfinite(float(loss)) and bool(torch.isfinite(g).all())
+    r = {"S": S, "K": st.K, "T": st.time_block, "dimx": dimx, "dimy": dimy, "elapsed": elapsed, "lat": lat,
+         "cells_step": 2.0 * S * dimx * dimy * nt, "h": h, "dt": dt, "nt": nt, "order": order, "nx": nx, "ny": ny}
     if timing:
         r["timing"] = st.timing()
         st.set_timing(False)
-    del model, st
+    del m, st, d_obs
+    torch.cuda.empty_cache()
+    return r
+
+
+def run_batched(c, K, steps, warmup, dev, time_block=1):
+    """Time `steps` misfit gradients of the batched engine on workload c."""
+    import torch
+    from torchwi_amd import Time2dBatch
+    order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
+    S = len(c["sx"])
+    model = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K,
+                        time_block=time_block)
+    vel = torch.from_numpy(c["vel"]).to(dev)
+    rng = np.random.default_rng(7)
+    vtrue = torch.from_numpy(c["vel"] * (1.0 + 0.02 * rng.standard_normal(c["vel"].shape)).astype(np.float32)).to(dev)
+    d_obs = model.modeling(vtrue, c["sx"], c["sy"], c["ry"]).clone()
+    for _ in range(warmup):
+        model.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        model.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    mg = model.plan.models[0]
+    cells = 2.0 * S * mg.dimx * mg.dimy * nt * steps
+    r = {"ckpt_every": model.plan.K, "time_block": model.plan.time_block,
+         "value": round(cells / elapsed / 1e9, 3), "ms_per_gradient": round(elapsed / steps * 1e3, 3),
+         "gradient_shots_per_s": round(S * steps / elapsed, 3),
+         "effective_hbm_frac": round(cells / elapsed * (BYTES_GRAD_STEP / 2.0) / (HBM_PEAK_GBS * 1e9), 4)}
+    del model, d_obs
     torch.cuda.empty_cache()
     return r
 
@@ -202,55 +282,53 @@ def run_plain(c, nshots, dev):
     el = time.perf_counter() - t0
     del model
     torch.cuda.empty_cache()
-    return {"gradient_shots_per_s": round(nshots / el, 3), "shots_timed": nshots,
-            "s_per_shot": round(el / nshots, 4)}
+    return {"gradient_shots_per_s": round(nshots / el, 3), "shots_timed": nshots, "s_per_shot": round(el / nshots, 4)}
 
 
-def run_scale(dev):
-    """Checkpointed misfit gradient of the 1000 x 3000 model (h = 10 m,
-    dt = 0.8 ms, nt = 8000, order 8, 64 ricker(6) shots, 3000 receivers),
-    where full storage (97 GB per shot) cannot be used: one untimed modelling
-    of the observed data, then one timed gradient.  The step kernel is chosen
-    by plan size as Time2dBatch does (the temporally blocked kernel here)."""
-    import torch
-    from torchwi_amd import Time2dBatch
+def run_compare(dev):
+    """The 384 x 1024 model, 32 ricker(8) shots, nt = 4000: batched gradient
+    with full storage (it fits: 212 GB) and checkpointed, against the plain
+    one-shot-per-launch form."""
     from torchwi_amd import workloads as W
-    from torchwi_amd.shots import auto_time_block, default_ckpt_interval
-    c = W.dipping_1000x3000(nshots=64, nt=8000)
-    order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
-    S = len(c["sx"])
-    tb = auto_time_block(S * (nx + order) * (ny + order))
-    K = default_ckpt_interval(nt, tb)
-    m = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K, time_block=tb)
-    vel = torch.from_numpy(c["vel"]).to(dev)
-    d_obs = m.modeling(vel * 1.01, c["sx"], c["sy"], c["ry"]).clone()
-    torch.cuda.synchronize()
+    from torchwi_amd.shots import default_ckpt_interval
+    c = W.dipping_384x1024()
+    full = run_batched(c, 0, 2, 1, dev)
+    ck = run_batched(c, default_ckpt_interval(c["nt"]), 1, 0, dev)
+    pl = run_plain(c, 2, dev)
+    return {"workload": "32 ricker(8) shots, 384x1024 model (padded 392x1032), h=12.5 m, dt=1 ms, nt=4000, order 8, "
+                        "one step per launch in hipGraph blocks of 100 steps",
+            "full_storage": full, "checkpointed": ck, "plain_one_shot": pl,
+            "speedup_vs_plain": {"full_storage": round(full["gradient_shots_per_s"] / pl["gradient_shots_per_s"], 2),
+                                 "checkpointed": round(ck["gradient_shots_per_s"] / pl["gradient_shots_per_s"], 2)}}
+
+
+def run_serving(n, dev):
+    """A short serving stream on the config-5 grids: n Poisson requests at
+    80% of the measured back-to-back capacity; p50/p99 latency."""
+    import torch
+    from torchwi_amd import serving as SV
+    kw = dict(seed=5, nt_range=(2000, 6000), shots_range=(1, 16))
+    probe = SV.make_stream(n, 1.0, **kw)
+    srv = SV.Server(dev, slots=4)
     t0 = time.perf_counter()
-    m.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
-    torch.cuda.synchronize()
-    t = time.perf_counter() - t0
-    cells = 2.0 * S * (nx + order) * (ny + order) * nt
-    v = cells / t / 1e9
-    del m, d_obs
+    srv.run(probe, realtime=False)
+    capacity = n / (time.perf_counter() - t0)
+    del srv, probe
     torch.cuda.empty_cache()
-    return {"workload": "checkpointed misfit gradient, %d shots, %dx%d model (padded %dx%d), h=%g m, dt=%g s, "
-                        "nt=%d, order %d, checkpoint every %d steps + recompute, %d steps per launch"
-                        % (S, ny, nx, ny + order, nx + order, h, dt, nt, order, K, tb),
-            "value": round(v, 3), "unit": "Gcell-updates/s", "s_per_gradient": round(t, 3),
-            "gradient_shots_per_s": round(S / t, 3), "ckpt_every": K, "time_block": tb,
-            "effective_hbm_frac": round(v * 1e9 * (BYTES_GRAD_STEP / 2.0) / (HBM_PEAK_GBS * 1e9), 4)}
-
-
-def load_traffic(mode_key, kernel_key):
-    """HBM bytes per launch of a step kernel at this workload, from the
-    calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs kept in profiles/
-    (mode_key: "full_storage" or "checkpointed")."""
-    path = os.path.join(ROOT, "profiles", "traffic_k_vstep_384x1024_32shots.json")
-    try:
-        with open(path) as f:
-            return json.load(f)[mode_key].get(kernel_key)
-    except (OSError, ValueError, KeyError):
-        return None
+    rate = 0.8 * capacity
+    reqs = SV.make_stream(n, rate, **kw)
+    srv = SV.Server(dev, slots=4)
+    t0 = time.perf_counter()
+    res = srv.run(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    out = {"workload": "%d Poisson requests (seed 5) at %.3f req/s (80%% of the %.3f req/s back-to-back capacity), "
+                       "grids %s, nt 2000-6000, 1-16 shots, 50/50 forward/gradient"
+                       % (n, rate, capacity, [list(g) for g in SV.GRIDS])}
+    out.update(SV.summarize(res, wall))
+    out["max_concurrent"] = srv.max_concurrent
+    del srv, reqs, res
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -261,81 +339,52 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
-    from torchwi_amd import workloads as W
-    from torchwi_amd.shots import default_ckpt_interval
 
     dev = torch.device("cuda:0")
     probe = copy_probe(args.copy_probe) if args.copy_probe else None
-    c = W.dipping_384x1024(nshots=args.nshots, nt=args.nt)
-    order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
-    tb = args.time_block
-    kck = args.ckpt_every or default_ckpt_interval(nt, tb)
-    K = {"full": 0, "ckpt": kck, "auto": "auto"}[args.mode]
-    gg = args.grad_group if tb == 1 else 1
-    r = run_batched(c, K, args.steps, args.warmup, not args.no_timing, args.graph_steps, dev, tb, args.tile_rows,
-                    args.tile_cols, gg)
-    S, kstate, value, elapsed, lat = r["S"], r["K"], r["value"], r["elapsed"], r["lat"]
+    r = run_scale(args, dev)
+    S, T, elapsed, lat = r["S"], r["T"], r["elapsed"], r["lat"]
+    value = r["cells_step"] * args.steps / elapsed / 1e9
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(elapsed / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "batched misfit gradient (forward + adjoint), %d shots, %dx%d model "
-                               "(padded %dx%d), h=%g m, dt=%g s, nt=%d, order %d, %s"
-                               % (S, ny, nx, r["dimy"], r["dimx"], h, dt, nt, order,
-                                  "full virtual-source storage" if kstate == 0 else "checkpoint every %d steps + recompute" % kstate),
-                   "shots": S, "nz": ny, "nx": nx, "nt": nt, "order": order, "ckpt_every": kstate,
-                   "graph_steps": args.graph_steps, "time_block": tb, "tile_rows": args.tile_rows,
-                   "tile_cols": args.tile_cols, "grad_group": gg},
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
-        "gradient_shots_per_s": round(r["shots_per_s"], 3),
+        "config": {"workload": "checkpointed batched misfit gradient (forward + adjoint), %d shots, %dx%d model "
+                               "(padded %dx%d), h=%g m, dt=%g s, nt=%d, order %d, checkpoint every %d steps + "
+                               "recompute, %d steps per launch"
+                               % (S, r["ny"], r["nx"], r["dimy"], r["dimx"], r["h"], r["dt"], r["nt"], r["order"],
+                                  r["K"], T),
+                   "shots": S, "nz": r["ny"], "nx": r["nx"], "nt": r["nt"], "order": r["order"],
+                   "ckpt_every": r["K"], "time_block": T, "tile_rows": args.tile_rows},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 1),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 1)},
+        "gradient_shots_per_s": round(S * args.steps / elapsed, 3),
+        # the survey's target metric: credited updates at 22 B each against 8 TB/s
         "effective_hbm_frac": round(value * 1e9 * (BYTES_GRAD_STEP / 2.0) / (HBM_PEAK_GBS * 1e9), 4),
     }
     if not args.no_timing:
-        tm = r["timing"]
-        per = {}
-        cells_launch = S * r["dimx"] * r["dimy"]
-        for kind, b in (("fwd", BYTES_FWD), ("fwd_virt", BYTES_FWD_VIRT), ("recomp", BYTES_FWD_VIRT), ("adj", BYTES_ADJ)):
-            ms, n = tm[kind]
-            if n:
-                avg = ms / n
-                per[kind] = {"avg_us": round(avg * 1e3, 3), "launches": n,
-                             "achieved_GBs": round(b * cells_launch / (avg * 1e-3) / 1e9, 1)}
+        cells = S * r["dimx"] * r["dimy"]
+        per = kernel_table(r["timing"], cells, T)
         out["kernels"] = per
-        dom = max(per, key=lambda k: per[k]["avg_us"] * per[k]["launches"])
-        b = {"fwd": BYTES_FWD, "fwd_virt": BYTES_FWD_VIRT, "recomp": BYTES_FWD_VIRT, "adj": BYTES_ADJ}[dom]
-        ach = per[dom]["achieved_GBs"]
-        tr = (load_traffic("full_storage" if kstate == 0 else "checkpointed", dom)
-              if (nt, S, order, tb, args.tile_rows, args.tile_cols, gg) == (4000, 32, 8, 1, 0, 0, 1) else None)
-        kname = ("k_pipe<4,%s,%d>" % (dom, tb) if tb > 1 else
-                 "k_vadj_group<4> (%d shots per group)" % gg if dom == "adj" and gg > 1 else "k_vstep<4,%s>" % dom)
-        out["roofline"] = {"bound": "hbm", "kernel": kname,
-                           "achieved": ach,
-                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                           "traffic": tr, "bytes_per_cell": b, "cells_per_launch": cells_launch}
-        if tr and probe:
-            # measured HBM bytes of the launch over its duration, against the
-            # streaming copy of the same device
-            real = tr["bytes_per_cell"] * cells_launch / (per[dom]["avg_us"] * 1e-6) / 1e9
-            out["roofline"]["traffic_GBs"] = round(real, 1)
-            out["roofline"]["frac_of_stream_copy"] = round(real / probe["stream_copy_GBs"], 4)
+        dom = max(per, key=lambda k: per[k]["share_of_time"])
+        kname = ("k_pipe<4,%s,%d>" % (dom, T)) if T > 1 else "k_vstep<4,%s>" % dom
+        out["roofline"] = {"bound": "hbm", "kernel": kname, "achieved": per[dom]["achieved_GBs"],
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": per[dom]["frac"], "traffic": None,
+                           "bytes_per_cell_per_launch": per[dom]["bytes_per_cell_per_launch"],
+                           "cells_per_launch": cells, "steps_per_launch": T}
+        tr = load_traffic("traffic_k_pipe_1000x3000_64shots.json")
+        if (tr and (S, r["nt"] >= 300, T, args.tile_rows) == (64, True, 3, 0)
+                and dom in tr.get("kernels", {})):
+            t = tr["kernels"][dom]
+            out["roofline"]["traffic"] = t["total_bytes"]
+            out["roofline"]["traffic_bytes_per_cell"] = t["bytes_per_cell"]
+            out["roofline"]["traffic_GBs"] = round(t["total_bytes"] / (per[dom]["avg_us_per_launch"] * 1e-6) / 1e9, 1)
     if args.compare:
-        if kstate == 0:
-            rc = run_batched(c, kck, 1, 0, False, args.graph_steps, dev, tb, args.tile_rows, args.tile_cols, gg)
-            out["checkpointed"] = {"ckpt_every": rc["K"], "value": round(rc["value"], 3),
-                                   "ms_per_step": round(rc["elapsed"] * 1e3, 3),
-                                   "gradient_shots_per_s": round(rc["shots_per_s"], 3),
-                                   "effective_hbm_frac": round(rc["value"] * 1e9 * (BYTES_GRAD_STEP / 2.0) / (HBM_PEAK_GBS * 1e9), 4)}
-            ck_rate = rc["shots_per_s"]
-        else:
-            ck_rate = r["shots_per_s"]
-        pl = run_plain(c, 2, dev)
-        out["plain_one_shot"] = pl
-        out["speedup_vs_plain"] = {"best": round(r["shots_per_s"] / pl["gradient_shots_per_s"], 2),
-                                   "checkpointed": round(ck_rate / pl["gradient_shots_per_s"], 2)}
-    if not args.no_scale:
-        out["checkpointed_at_scale"] = run_scale(dev)
+        out["batched_384x1024"] = run_compare(dev)
+    if args.serving_requests:
+        out["serving"] = run_serving(args.serving_requests, dev)
     if probe:
         out["copy_probe"] = probe
     if not args.no_cpu_baseline:

@@ -9,7 +9,10 @@ known number of bytes through a plain device copy in the same process; its
 counter readings give the factor between each counter and real bytes, which
 is then applied to the step kernels.  Kernel kinds are recognised from the
 step-flag template argument of their names (3 forward, 7 forward with
-virtual-source storage, 5 recomputation, 24 adjoint).
+virtual-source storage, 5 recomputation, 24 adjoint); the temporally blocked
+kernel (``k_pipe<NE, FLAGS, T>``) is compared with its own compulsory bytes
+per launch, the one-step kernel (``k_vstep``) with the per-unit bytes of one
+cell-update.
 """
 import csv
 import json
@@ -17,8 +20,16 @@ import re
 import sys
 
 KIND_BY_FLAGS = {3: "fwd", 7: "fwd_virt", 5: "recomp", 24: "adj"}
+# one-step kernels: algorithmic bytes per cell-update
 ALGORITHMIC = {"fwd": 16, "fwd_virt": 20, "recomp": 20, "adj": 28}
-_FLAGS = re.compile(r"k_(?:vstep|pipe)<\d+, (\d+)u")
+_NAME = re.compile(r"k_(vstep|pipe)<\d+, (\d+)u(?:, (\d+))?")
+
+
+def pipe_algorithmic(kind, T):
+    """Compulsory bytes per cell of one blocked launch of T steps (fields in
+    and out once, model arrays, T virtual sources, one gradient
+    read-modify-write)."""
+    return {"fwd": 20, "fwd_virt": 24 + 4 * T, "recomp": 24 + 4 * T, "adj": 28 + 4 * T}[kind]
 
 
 def _rows(path, counter):
@@ -30,6 +41,18 @@ def _rows(path, counter):
     return out
 
 
+def _kind(name):
+    """(kind, steps per launch or 0 for the one-step kernel) of a step-kernel
+    name, or None."""
+    m = _NAME.search(name)
+    if not m:
+        return None
+    kind = KIND_BY_FLAGS.get(int(m.group(2)))
+    if kind is None:
+        return None
+    return kind, (int(m.group(3)) if m.group(1) == "pipe" and m.group(3) else 0)
+
+
 def _per_kind(rows):
     """Mean counter value per dispatch of each step-kernel kind.  Graph nodes
     that lie past the end of the time loop (the tail of a partial block or
@@ -37,17 +60,14 @@ def _per_kind(rows):
     of the kind's median are those and are left out."""
     vals = {}
     for name, v, grid in rows:
-        m = _FLAGS.search(name)
-        if not m:
-            continue
-        kind = KIND_BY_FLAGS.get(int(m.group(1)))
-        if kind is not None:
-            vals.setdefault(kind, []).append(v)
+        k = _kind(name)
+        if k is not None:
+            vals.setdefault(k, []).append(v)
     out = {}
-    for kind, v in vals.items():
+    for k, v in vals.items():
         med = sorted(v)[len(v) // 2]
         keep = [x for x in v if x >= 0.1 * med]
-        out[kind] = sum(keep) / len(keep)
+        out[k] = sum(keep) / len(keep)
     return out
 
 
@@ -66,11 +86,14 @@ def summarize(fetch_csv, write_csv, copy_mib, cells_per_launch, workload):
     ff, wf = copy_bytes / (fc * 1024.0), copy_bytes / (wc * 1024.0)
     fk, wk = _per_kind(fr), _per_kind(wr)
     kernels = {}
-    for kind in sorted(set(fk) & set(wk)):
-        rb, wb = fk[kind] * 1024.0 * ff, wk[kind] * 1024.0 * wf
+    for key in sorted(set(fk) & set(wk)):
+        kind, T = key
+        rb, wb = fk[key] * 1024.0 * ff, wk[key] * 1024.0 * wf
+        alg = pipe_algorithmic(kind, T) if T else ALGORITHMIC[kind]
         kernels[kind] = {"read_bytes": int(rb), "write_bytes": int(wb), "total_bytes": int(rb + wb),
                          "bytes_per_cell": round((rb + wb) / cells_per_launch, 2),
-                         "algorithmic_bytes_per_cell": ALGORITHMIC[kind]}
+                         "algorithmic_bytes_per_cell": alg,
+                         "steps_per_launch": T or 1}
     return {"workload": workload,
             "calibration": {"copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes,
                             "FETCH_SIZE_KiB": fc, "WRITE_SIZE_KiB": wc,
