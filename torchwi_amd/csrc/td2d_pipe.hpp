@@ -70,10 +70,13 @@ struct PLayout {
     static constexpr int CUR = 0;                          // ring k: NCUR rows of level k
     static constexpr int PRV = CUR + T * NCUR * PLANES;    // ring k: NPRV rows of level k-1
     static constexpr int GRD = PRV + T * NPRV * PLANES;    // ring k-1: NGRD rows of gradient sums after level k
-    static constexpr int KS = GRD + ((FLAGS & F_GRAD) != 0 ? (T - 1) * NGRD * PLANES : 0);  // per level: Keys scratch
+    // adjoint: ring k-1 holds the model rows of level k, passed to level k+1
+    // (read LAG iterations after the write, like the u_prev ring)
+    static constexpr int MDL = GRD + ((FLAGS & F_GRAD) != 0 ? (T - 1) * NGRD * PLANES : 0);
+    static constexpr int KS = MDL + ((FLAGS & F_GRAD) != 0 ? (T - 1) * NPRV * PLANES : 0);  // per level: Keys scratch
     static constexpr int SIZE0 = KS + T * 32;
-    // The adjoint's LDS is padded to 41.6 KB so that at most 3 workgroups (3
-    // waves per SIMD) share a CU: at 4 (its 126 VGPRs would allow them) it
+    // The adjoint's LDS is at least 41.6 KB so that at most 3 workgroups (3
+    // waves per SIMD) share a CU: at 4 (its VGPRs would allow them) it
     // measured 4% slower at 64 shots of 3008 x 1008 and 10% slower at 192
     // shots of 1032 x 392.
     static constexpr int SIZE = ((FLAGS & F_GRAD) != 0 && SIZE0 < 2600) ? 2600 : SIZE0;
@@ -201,7 +204,9 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
         constexpr int ROLE = decltype(RC)::value;
         const int row = O + slot_i - LAG * k;
         const bool rowA = row >= aLo && row < aHi, rowB = row >= bLo && row < bHi;
-        A = gld4(pA + ((aLane && rowA) ? row * ld + gx : 0));
+        // adjoint level waves after the first take the model row from the
+        // previous level through LDS
+        if constexpr ((FLAGS & F_GRAD) == 0 || ROLE != 2) A = gld4(pA + ((aLane && rowA) ? row * ld + gx : 0));
         if constexpr ((FLAGS & F_GRAD) != 0) {
             const int ib = (useB && bLane && rowB) ? row * ld + gx : 0;
             if constexpr (ROLE == 0) B = gld4(pB + ib);
@@ -256,6 +261,8 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
     const int prvW = LY::PRV + k * NPRV * PLANES + lane;
     const int grdR = LY::GRD + (k > 1 ? k - 2 : 0) * NGRD * PLANES + lane;   // sum after level k-1
     const int grdW = LY::GRD + (k > 0 ? k - 1 : 0) * NGRD * PLANES + lane;   // sum after level k
+    const int mdlR = LY::MDL + (k > 1 ? k - 2 : 0) * NPRV * PLANES + lane;   // model rows of level k-1
+    const int mdlW = LY::MDL + (k > 0 ? k - 1 : 0) * NPRV * PLANES + lane;   // model rows of level k
     const bool writes_next = k < T && k < a.nsteps;
     const int ksb = LY::KS + (k > 0 ? k - 1 : 0) * 32;   // this level's Keys scratch (float4 index)
 
@@ -293,11 +300,17 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
             // would become four conditional 4-byte reads
             float4 Gk = make_float4(0.f, 0.f, 0.f, 0.f);
             if constexpr ((FLAGS & F_GRAD) != 0 && ROLE != 1) Gk = S[grdR + cPr * PLANES];
+            // model of row r: wave 1 of the adjoint (and every level wave of
+            // the forward steps) from its own prefetch queue, the later
+            // adjoint level waves from wave k-1 (written with its u_prev row)
+            float4 Mk;
+            if constexpr ((FLAGS & F_GRAD) != 0 && ROLE == 2) Mk = S[mdlR + cPr * PLANES];
+            else Mk = qA[U % PQ];
             const int r = O + i - LAG * k;
             if (active && r >= Ok && r < Ek) {
                 const float4 C = win[(U + PW - 4) % PW];
                 const float4 P = Pk;
-                const float4 M = qA[U % PQ];
+                const float4 M = Mk;
                 float4 un;
                 if (r < NE) {
                     un = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -445,6 +458,7 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
                 if (writes_next) {
                     S[curW + cC * PLANES] = un;
                     S[prvW + cP * PLANES] = C;
+                    if constexpr ((FLAGS & F_GRAD) != 0) S[mdlW + cP * PLANES] = M;
                 }
             }
         }
