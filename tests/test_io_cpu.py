@@ -1,11 +1,12 @@
 """RSF files and shot-gather files (CPU).
 
-The writer's header text is fixed by the reference (torchwi/io/rsf.py:369-393):
-``in``, ``esize``, ``data_format`` on lines of their own (string values
-quoted), then one line per axis ``n# o# d#`` with ``label#`` / ``unit#``
-only when set.  The reader takes the last ``key=value`` of each key anywhere
-in the text (rsf.py:313-324), which is how Madagascar headers accumulate
-history; the data file is read n1 fastest.
+The header text a writer emits is RSF's canonical form (the one the
+reference's writer produces too, torchwi/io/rsf.py:369-393): ``in``,
+``esize``, ``data_format`` on lines of their own (string values quoted),
+then one line per axis ``n# o# d#`` with ``label#`` / ``unit#`` only when
+set.  A reader takes the last ``key=value`` of each key anywhere in the text
+(rsf.py:313-324), which is how Madagascar headers accumulate history; the
+data file is read n1 fastest.
 """
 import os
 
@@ -71,19 +72,27 @@ def test_ascii_form(tmp_path):
     assert np.array_equal(rsf.fromfile(f, "data"), a)
 
 
-def test_errors_exit_like_the_reference(tmp_path, capsys):
+def test_errors_raise(tmp_path):
     f = tmp_path / "noin.rsf"
     f.write_text("n1=3 d1=1\n")
-    with pytest.raises(SystemExit) as e:
+    with pytest.raises(rsf.RsfError, match="in="):
         rsf.fromfile(str(f), "n1")
-    assert e.value.code == 1 and "Cannot find 'in'" in capsys.readouterr().err
     f.write_text('n1=3 in="%s"\n' % (tmp_path / "missing@"))
-    with pytest.raises(SystemExit):
+    with pytest.raises(rsf.RsfError, match="does not exist"):
         rsf.fromfile(str(f), "n1")
-    with pytest.raises(SystemExit):
+    with pytest.raises(rsf.RsfError, match="unknown header key"):
         rsf.output(str(tmp_path / "x.rsf"), bogus=3)
-    with pytest.raises(SystemExit):
-        rsf.RSF(data_format="native_char").data_format = "native_char"
+    with pytest.raises(rsf.RsfError, match="data_format"):
+        rsf.output(str(tmp_path / "x.rsf"), data_format="native_char")
+    f.write_text('n1=x in="%s"\n' % f)
+    with pytest.raises(rsf.RsfError, match="bad value"):
+        rsf.fromfile(str(f), "n1")
+    # a short data file is an error, not a silently truncated array
+    rsf.tofile(str(tmp_path / "s.rsf"), np.zeros(4, np.float32), n1=4)
+    txt = (tmp_path / "s.rsf").read_text().replace("n1=4", "n1=5")
+    (tmp_path / "s.rsf").write_text(txt)
+    with pytest.raises(rsf.RsfError, match="holds 4 of 5"):
+        rsf.fromfile(str(tmp_path / "s.rsf"), "data")
 
 
 def test_shot_gathers_and_datasets(tmp_path):

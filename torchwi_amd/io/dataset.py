@@ -1,96 +1,112 @@
-"""Time-domain shot datasets and shot-gather files.
+"""Shot tables and per-shot gather files on either side of the propagator.
 
-``TimeDataset`` / ``TimeForwardDataset`` follow the reference's datasets
-(torchwi/io/Dataset.py:6-66): items are ``(sx, sy, data[, index])`` and
-``(sx, sy[, index])``; observed data are read per shot from ``ftrue % index``
-as raw float32 ('bin'/'binary') or ``.npy`` ('npy').  The reference's 'su'
-reader depends on an external package (pkrh) and is not provided; asking for
-it raises.  Deviation: a ``shape`` given to ``TimeDataset`` is applied (the
-reference assigns an undefined name there, Dataset.py:43).
+A survey on disk is a table of source positions plus one gather file per
+shot, named by a printf pattern (``seismo.%04d``).  Each file holds that
+shot's ``(nt, nx)`` float32 gather, as raw machine-order floats (``bin``,
+what ``write_gather`` and the reference's forward-modelling example write,
+examples/time/forward_modeling.py:52) or as a NumPy ``.npy`` file (``npy``).
 
-``write_gather`` / ``read_gathers`` are the per-shot gather files the
-reference's forward-modelling example writes (``seismo.%04d``: the (nt, nx)
-float32 gather, ``tofile``; examples/time/forward_modeling.py:52), stacked
-for the batched engine.
+``GatherFiles`` reads such files.  ``read_gathers`` stacks several shots for
+the batched engine.  ``TimeDataset`` and ``TimeForwardDataset`` are the
+``torch.utils.data`` views that TorchWI's loops iterate
+(torchwi/io/Dataset.py; examples/time/inv_torch.py):
+``TimeDataset(sxy, pattern, with_index=False, data_type='bin', shape=None,
+transpose=False)`` yields ``(sx, sy, gather[, index])`` and
+``TimeForwardDataset(sxy, with_index=True)`` yields ``(sx, sy[, index])``.
+Seismic Unix files (``su``) need an external reader that is not part of this
+package; asking for them raises ``NotImplementedError``.
 """
 import numpy as np
 import torch
 
 
+class GatherFiles:
+    """Per-shot gather files ``pattern % index`` of one storage kind.
+
+    kind: "bin" (alias "binary") raw float32, or "npy".  shape: the (nt, nx)
+    each raw file must hold (checked; None keeps raw files flat).
+    transpose: return (nx, nt) instead of (nt, nx)."""
+
+    KINDS = ("bin", "binary", "npy")
+
+    def __init__(self, pattern, kind="bin", shape=None, transpose=False):
+        if kind == "su":
+            raise NotImplementedError("Seismic Unix gathers need an external SU reader, not part of torchwi_amd")
+        if kind not in self.KINDS:
+            raise ValueError("unknown gather file kind %r (use %s)" % (kind, ", ".join(self.KINDS)))
+        self.pattern = pattern
+        self.kind = "bin" if kind == "binary" else kind
+        self.shape = tuple(shape) if shape is not None else None
+        self.transpose = bool(transpose)
+
+    def path(self, index):
+        return self.pattern % index
+
+    def load_array(self, index):
+        p = self.path(index)
+        if self.kind == "npy":
+            a = np.load(p, allow_pickle=False)
+        else:
+            a = np.fromfile(p, dtype=np.float32)
+            if self.shape is not None:
+                want = int(np.prod(self.shape))
+                if a.size != want:
+                    raise ValueError("%s holds %d values, expected %s = %d" % (p, a.size, self.shape, want))
+                a = a.reshape(self.shape)
+        return a.T if self.transpose else a
+
+    def load(self, index):
+        return torch.from_numpy(np.ascontiguousarray(self.load_array(index)))
+
+
 class TimeDataset(torch.utils.data.Dataset):
-    def __init__(self, sxy, ftrue, with_index=False, data_type='bin', shape=None, transpose=False):
+    """Shots with observed data: item i is (sx, sy, gather) or, with
+    with_index, (sx, sy, gather, i).  sxy: sequence of (sx, sy) positions."""
+
+    def __init__(self, sxy, ftrue, with_index=False, data_type="bin", shape=None, transpose=False):
         super().__init__()
-        self.ftrue = ftrue
-        self.with_index = with_index
         self.sxy = sxy
-        self.data_type = data_type
-        self.shape = shape
-        self.transpose = transpose
+        self.with_index = bool(with_index)
+        self.files = GatherFiles(ftrue, data_type, shape, transpose)
 
     def __len__(self):
         return len(self.sxy)
 
     def __getitem__(self, index):
-        truedata = self.load_data(index)
-        sx, sy = self.sxy[index]
-        if self.with_index:
-            return sx, sy, truedata, index
-        return sx, sy, truedata
-
-    def load_data(self, index):
-        if self.data_type in ('bin', 'binary'):
-            truedata = self.load_bin(index)
-        elif self.data_type == 'npy':
-            truedata = self.load_npy(index)
-        elif self.data_type == 'su':
-            raise NotImplementedError("SU gathers need the external 'pkrh' reader, which is not part of this package")
-        else:
-            raise ValueError("unknown data_type %r (bin, binary, npy)" % (self.data_type,))
-        if self.transpose:
-            truedata = truedata.T
-        return torch.from_numpy(np.ascontiguousarray(truedata))
-
-    def load_bin(self, index):
-        truedata = np.fromfile(self.ftrue % index, dtype=np.float32)
-        if self.shape:
-            truedata.shape = self.shape
-        return truedata
-
-    def load_npy(self, index):
-        return np.load(self.ftrue % index, allow_pickle=False)
+        pos = tuple(self.sxy[index])
+        item = pos + (self.files.load(index),)
+        return item + (index,) if self.with_index else item
 
 
 class TimeForwardDataset(torch.utils.data.Dataset):
+    """Shots to model: item i is (sx, sy, i), or (sx, sy) without with_index."""
+
     def __init__(self, sxy, with_index=True):
         super().__init__()
-        self.with_index = with_index
         self.sxy = sxy
-        self.nshot = len(sxy)
+        self.with_index = bool(with_index)
 
     def __len__(self):
-        return self.nshot
+        return len(self.sxy)
 
     def __getitem__(self, index):
-        sx, sy = self.sxy[index]
-        if self.with_index:
-            return sx, sy, index
-        return sx, sy
+        pos = tuple(self.sxy[index])
+        return pos + (index,) if self.with_index else pos
 
 
 def write_gather(pattern, index, gather):
     """Write one shot's (nt, nx) gather as raw float32 to ``pattern % index``."""
-    g = gather.detach().to('cpu').numpy() if isinstance(gather, torch.Tensor) else np.asarray(gather)
+    g = gather.detach().to("cpu").numpy() if isinstance(gather, torch.Tensor) else np.asarray(gather)
     np.ascontiguousarray(g, dtype=np.float32).tofile(pattern % index)
 
 
 def read_gathers(pattern, indices, nt, nrec, device=None):
     """Stack the raw float32 gathers ``pattern % i`` (each nt x nrec) of the
     given shots into one (S, nt, nrec) tensor, on ``device`` if given."""
+    files = GatherFiles(pattern, "bin", (nt, nrec))
+    indices = list(indices)
     out = np.empty((len(indices), nt, nrec), np.float32)
     for s, i in enumerate(indices):
-        g = np.fromfile(pattern % i, dtype=np.float32)
-        if g.size != nt * nrec:
-            raise ValueError("%s holds %d values, expected nt*nrec = %d" % (pattern % i, g.size, nt * nrec))
-        out[s] = g.reshape(nt, nrec)
+        out[s] = files.load_array(i)
     t = torch.from_numpy(out)
     return t.to(device) if device is not None else t

@@ -1,379 +1,296 @@
-"""RSF (Madagascar) files: a text header and a binary or ASCII data file.
+"""RSF files (Madagascar's regularly sampled format): a text header plus the
+data file it points to.
 
-The interface follows the reference's reader/writer (torchwi/io/rsf.py): the
-module functions ``input``, ``output``, ``fromfile``, ``tofile`` and the
-class ``RSF`` with ``get``/``gets``/``read``/``traces``/``write``, the same
-header keys, the same header text on write, and the same error behaviour (a
-message on stderr and exit status 1, i.e. ``SystemExit``).  Differences: it
-runs on NumPy 2 (the reference's ``np.product``, rsf.py:224, is gone there),
-and ``__setitem__`` works (rsf.py:163-165 refers to an undefined name).
+The format, as this module reads and writes it:
 
-Header grammar (rsf.py:313-324, 369-382): ``key=value`` pairs anywhere in
-the text, the LAST occurrence of a key wins, values may be quoted with ' or
-".  Integer keys: esize, n1..n9; float keys: o1..o9, d1..d9.  The data file
-holds n1 values fastest; ``read(read_all=True)`` returns shape
-(n_dim, ..., n2, n1).  ``data_format`` is ``<form>_<type>`` with form
-native|ascii and type float|int|complex (float32, int32, complex64).
+* The header is free text made of ``key=value`` tokens.  Tokens may sit
+  anywhere, among the history lines earlier programs appended, and
+  whitespace may surround ``=``.  A value is a double- or single-quoted
+  string (which may hold blanks) or a run of non-blank characters.  When a
+  key occurs more than once, its last occurrence is the one in force.
+* ``in`` names the data file.  ``data_format`` is ``<form>_<type>``, with form
+  ``native`` (raw machine-order binary) or ``ascii`` (a text table), and type
+  ``float`` (4-byte), ``int`` (4-byte) or ``complex`` (2 x 4-byte).  ``esize``
+  is the bytes per element (0 for ascii).
+* Axis k (k = 1 is the fastest) is described by ``nk`` (samples), ``ok``
+  (origin), ``dk`` (sampling), ``labelk`` and ``unitk``.  The data array has
+  shape ``(n_N, ..., n2, n1)``, N being the highest axis with ``nN`` set.
+
+A header written here is the canonical form that RSF tools and the
+reference's writer emit: ``in``, ``esize`` and ``data_format`` on lines of
+their own (strings quoted), then one line per axis ``nk= ok= dk=``, with
+``labelk`` / ``unitk`` only when set.  Errors raise ``RsfError``.
+
+Interface (the names TorchWI's examples call, torchwi/io/rsf.py):
+``fromfile(path, keys)``, ``tofile(path, data, **keys)``, ``input(path)`` (a
+reader: ``r[key]``, ``r.shape``, ``r.read()``, ``r.traces()``) and
+``output(path, **keys)`` (a writer: ``w.write(data)``).
 """
 import os
 import re
-import sys
-import tempfile
 
 import numpy as np
 
-MAXDIM = 9
-RSFKEYS_READ = (['in', 'esize', 'data_format', 'type', 'form', '_in', 'infile'] +
-                [k + str(i) for i in range(1, MAXDIM + 1) for k in ('n', 'o', 'd', 'label', 'unit')])
-RSFKEYS_WRITE = RSFKEYS_READ[:3]
-KEY_INT = ['esize'] + ['n%d' % i for i in range(1, MAXDIM + 1)]
-KEY_FLOAT = [k + str(i) for i in range(1, MAXDIM + 1) for k in ('o', 'd')]
-_AXKEYS = ['ax%d' % i for i in range(1, MAXDIM + 1)] + ['axes']
-_DTYPES = {'float': np.float32, 'int': np.int32, 'complex': np.complex64}
+MAX_AXES = 9
+_TYPES = {"float": np.dtype(np.float32), "int": np.dtype(np.int32), "complex": np.dtype(np.complex64)}
+_FORMS = ("native", "ascii")
+_AXIS_FIELDS = ("n", "o", "d", "label", "unit")
+# a key, '=', then a quoted string or a run of non-blank characters
+_TOKEN = re.compile(r"""([A-Za-z_][A-Za-z0-9_]*)\s*=\s*("[^"]*"|'[^']*'|\S+)""")
+_AXIS_KEY = re.compile(r"^(n|o|d|label|unit)([1-9])$")
 
 
-def warn(msg):
-    sys.stderr.write(msg + "\n")
+class RsfError(ValueError):
+    """A malformed header, a missing data file or an unsupported key/format."""
 
 
-def errexit(msg, errno=1):
-    """Report and exit with status errno, as the reference does (rsf.py:400)."""
-    warn(msg)
-    sys.exit(errno)
+def parse_header(text):
+    """All key=value pairs of a header text as strings (quotes removed), the
+    last occurrence of each key winning."""
+    keys = {}
+    for key, val in _TOKEN.findall(text):
+        if len(val) >= 2 and val[0] == val[-1] and val[0] in "\"'":
+            val = val[1:-1]
+        keys[key] = val
+    return keys
 
 
-def quote(s):
-    return '"' + s + '"'
+def _split_format(fmt):
+    form, _, typ = str(fmt).lower().partition("_")
+    if form not in _FORMS or typ not in _TYPES:
+        raise RsfError("unsupported data_format %r (use native|ascii _ float|int|complex)" % (fmt,))
+    return form, typ
 
 
-def unquote(s):
-    if s.startswith("'") or s.startswith('"'):
-        return s[1:-1]
-    return s
+def _axis_value(field, text):
+    """Typed value of an axis field given as text."""
+    if field == "n":
+        return int(text)
+    if field in ("o", "d"):
+        return float(text)
+    return str(text)
 
 
-def _check_key(key):
-    if key not in RSFKEYS_READ + _AXKEYS:
-        print("Wrong keyword:", key)
-        sys.exit(1)
+class Header:
+    """A parsed header: the data file, its format and the axes."""
+
+    def __init__(self, infile="", data_format="native_float", axes=None):
+        self.infile = infile
+        self.form, self.type = _split_format(data_format)
+        # axes[k-1] = dict(n, o, d, label, unit) of axis k
+        self.axes = [dict(n=0, o=0.0, d=0.0, label="", unit="") for _ in range(MAX_AXES)]
+        for k, ax in enumerate(axes or []):
+            self.axes[k].update(ax)
+
+    @classmethod
+    def from_text(cls, text):
+        kv = parse_header(text)
+        h = cls(kv.get("in", ""), kv.get("data_format", "native_float"))
+        for key, val in kv.items():
+            m = _AXIS_KEY.match(key)
+            if m:
+                try:
+                    h.axes[int(m.group(2)) - 1][m.group(1)] = _axis_value(m.group(1), val)
+                except ValueError:
+                    raise RsfError("bad value %s=%s" % (key, val)) from None
+        return h
+
+    @property
+    def data_format(self):
+        return "%s_%s" % (self.form, self.type)
+
+    @property
+    def dtype(self):
+        return _TYPES[self.type]
+
+    @property
+    def esize(self):
+        return 0 if self.form == "ascii" else self.dtype.itemsize
+
+    @property
+    def ndim(self):
+        return max((k + 1 for k, ax in enumerate(self.axes) if ax["n"] > 0), default=0)
+
+    @property
+    def shape(self):
+        """(n_N, ..., n1): slowest axis first."""
+        return tuple(self.axes[k]["n"] for k in range(self.ndim - 1, -1, -1))
+
+    def value(self, key):
+        if key == "in":
+            return self.infile
+        if key == "data_format":
+            return self.data_format
+        if key == "esize":
+            return self.esize
+        if key in ("form", "type"):
+            return getattr(self, key)
+        m = _AXIS_KEY.match(key)
+        if not m:
+            raise RsfError("unknown header key %r" % (key,))
+        return self.axes[int(m.group(2)) - 1][m.group(1)]
+
+    def text(self):
+        """Canonical header text."""
+        lines = ['in="%s"' % self.infile, "esize=%d" % self.esize, 'data_format="%s"' % self.data_format]
+        for k in range(self.ndim):
+            ax = self.axes[k]
+            line = "n%d=%d o%d=%r d%d=%r" % (k + 1, ax["n"], k + 1, float(ax["o"]), k + 1, float(ax["d"]))
+            for f in ("label", "unit"):
+                if ax[f]:
+                    line += ' %s%d="%s"' % (f, k + 1, ax[f])
+            lines.append(line)
+        return "\n".join(lines) + "\n"
 
 
-def _set_type(key, val, quote_str=False):
-    if quote_str:
-        return val if (key in KEY_INT or key in KEY_FLOAT) else quote(val)
-    if key in KEY_INT:
-        return int(val)
-    if key in KEY_FLOAT:
-        return float(val)
-    return unquote(val)
+class Reader:
+    """An RSF file open for reading (``input``)."""
 
-
-def fromstring(key, text, un_quote=True):
-    """Value of the last ``key=value`` in text: (value, True), or (0, False)."""
-    vals = re.findall(re.escape(key) + r"\s*=\s*(\S+)", text)
-    if not vals:
-        return 0, False
-    val = vals[-1]
-    for q in ("'", '"'):
-        if val.startswith(q) and not val.endswith(q):
-            full = re.findall(re.escape(key) + "=" + q + "(.+?)" + q, text)
-            if full:
-                val = full[-1]
-    if un_quote:
-        val = unquote(val)
-    return val, True
-
-
-def file2str(filename=''):
-    if filename and filename.lower() != 'stdin':
-        with open(filename, 'r') as fh:
-            return fh.read()
-    return sys.stdin.read()
-
-
-def str2file(string, filename=''):
-    if filename and filename.lower() != 'stdout':
-        with open(filename, 'w') as fh:
-            fh.write(string)
-    else:
-        sys.stdout.write(string)
-
-
-class Axis:
-    def __init__(self, n=0, o=0., d=0., label='', unit=''):
-        self.n = int(n)
-        self.o = float(o)
-        self.d = float(d)
-        self.label = unquote(label)
-        self.unit = unquote(unit)
-
-    def to_str(self, idim):
-        s = 'n%d=%d o%d=%s d%d=%s' % (idim, self.n, idim, self.o, idim, self.d)
-        if self.label:
-            s += ' label%d="%s"' % (idim, self.label)
-        if self.unit:
-            s += ' unit%d="%s"' % (idim, self.unit)
-        return s
-
-    def __repr__(self):
-        return "Axis(n=%d, o=%s, d=%s, label=%r, unit=%r)" % (self.n, self.o, self.d, self.label, self.unit)
-
-
-class RSF:
-    def __init__(self, mode=None, readonly=True, data_format=None):
-        self._in = ""
-        self._open = False
-        self.f = None
-        self.filename = ''
-        self.wrote = False
-        self._data_format = data_format or 'native_float'
-        self._mode = mode or self._guess_mode(readonly)
-        self.axes = [Axis() for _ in range(MAXDIM)]
-
-    def _guess_mode(self, readonly):
-        if readonly:
-            return 'r' if self.form == 'ascii' else 'rb'
-        return 'wb'
-
-    def open_in(self, tmpfile=False, abspath=True):
-        if tmpfile:
-            self.f = tempfile.NamedTemporaryFile(mode=self._mode, suffix=".rsf@", dir="./", delete=False)
-            self.set_in(self.f.name, abspath)
-        else:
-            self.f = open(self._in, self._mode)
-        self._open = True
+    def __init__(self, path):
+        self.path = path
+        with open(path) as f:
+            self.header = Header.from_text(f.read())
+        if not self.header.infile:
+            raise RsfError("%s: the header has no in= (data file) entry" % path)
+        if not os.path.exists(self.header.infile):
+            raise RsfError("%s: data file %s does not exist" % (path, self.header.infile))
+        self._pos = 0      # elements consumed by read(count) / traces()
 
     def __enter__(self):
         return self
 
-    def __exit__(self, exception_type, exception_value, traceback):
-        self.close()
+    def __exit__(self, *exc):
+        return False
 
-    # ---- header access -----------------------------------------------------
-    def get(self, key):
-        if key == 'in':
-            return self._in
-        if key in ('esize', 'data_format', 'form', 'type'):
-            return getattr(self, key)
-        if key == 'axes':
-            return self.axes
-        if key.startswith('ax'):
-            return self.axes[int(key[-1]) - 1]
-        if key == 'data':
-            return self.read(read_all=True)
-        if key == 'header':
-            return self.header_dict()
-        name, idim = key[:-1], int(key[-1])
-        return getattr(self.axes[idim - 1], name)
+    def close(self):
+        pass
 
     def __getitem__(self, key):
-        return self.get(key)
-
-    def __setitem__(self, key, value):
-        self.set(**{key: value})
-
-    def header_dict(self, contain_in=False):
-        dic = {}
-        if contain_in:
-            dic['in'] = self._in
-        dic.update(self.axes_dict())
-        dic['data_format'] = self.data_format
-        dic['esize'] = self.esize
-        return dic
-
-    def gets(self, keys):
-        if len(keys) == 1:
-            return self.get(keys[0])
-        return [self.get(k) for k in keys]
-
-    def axes_dict(self):
-        return {"ax%d" % i: self.get("ax%d" % i) for i in range(1, self.dimension + 1)}
-
-    def set_in(self, _in, abspath=True):
-        self._in = os.path.abspath(_in) if abspath else _in
-
-    def set(self, **kwargs):
-        for key, val in kwargs.items():
-            if key in ('in', 'infile', '_in'):
-                self._in = val
-            elif key == 'esize':
-                self._esize = val
-            elif key in ('data_format', 'form', 'type'):
-                setattr(self, key, val)
-            elif key == 'axes':
-                for idim, ax in enumerate(val):
-                    self.axes[idim] = ax
-            elif key.startswith('ax'):
-                self.axes[int(key[-1]) - 1] = val
-            else:
-                name, idim = key[:-1], int(key[-1])
-                setattr(self.axes[idim - 1], name, val)
-
-    def size(self, istart=0):
-        """Product of shape[istart:], shape = (n_dim, ..., n2, n1): all values
-        for 0, the values of one slice of the slowest axis for 1 (the
-        reference's code, rsf.py:216-224; its docstring counts the other way)."""
-        return int(np.prod(self.shape[istart:], dtype=np.int64))
-
-    @property
-    def dimension(self):
-        dim = 0
-        for idim in range(1, MAXDIM + 1):
-            if self.get("n%d" % idim) > 0:
-                dim = idim
-        return dim
+        return self.data if key == "data" else self.header.value(key)
 
     @property
     def shape(self):
-        return tuple(self.get("n%d" % idim) for idim in range(self.dimension, 0, -1))
+        return self.header.shape
 
     @property
-    def dtype(self):
-        if self.type not in _DTYPES:
-            print("Wrong type:", self.type)
-            sys.exit(1)
-        return _DTYPES[self.type]
-
-    @property
-    def data_format(self):
-        return self._data_format
-
-    @data_format.setter
-    def data_format(self, val):
-        self._data_format = val
-        if self.type.lower() not in ('int', 'float', 'complex'):
-            print('Wrong type:', self.type)
-            sys.exit(1)
-        if self.form.lower() not in ('native', 'ascii'):
-            print('Wrong format:', self.form)
-            sys.exit(1)
-
-    @property
-    def type(self):
-        return self._data_format.split('_')[1].lower()
-
-    @type.setter
-    def type(self, val):
-        self._data_format = "%s_%s" % (self._data_format.split('_')[0], val)
-
-    @property
-    def form(self):
-        return self._data_format.split('_')[0].lower()
-
-    @form.setter
-    def form(self, val):
-        self._data_format = "%s_%s" % (val, self._data_format.split('_')[1])
+    def dimension(self):
+        return self.header.ndim
 
     @property
     def esize(self):
-        if self.form == 'ascii':
-            return 0
-        return 8 if self.type == 'complex' else 4
+        return self.header.esize
 
-    # ---- data --------------------------------------------------------------
-    def close(self):
-        if self._open:
-            self.f.close()
-            self._open = False
+    @property
+    def dtype(self):
+        return self.header.dtype
 
-    def read(self, read_all=False):
-        """All values (read_all, shaped and closing the file) or the next
-        trace of n1 values; native or ASCII."""
-        if self.form == "native":
-            count = self.size() if read_all else self.get("n1")
-            data = np.fromfile(self.f, dtype=self.dtype, count=count)
-            if read_all:
-                data.shape = self.shape
-                self.close()
-        else:
-            # the writer prints every type with a float format ('%.18e'), so
-            # integers are read as doubles and converted
-            data = np.loadtxt(self.f, dtype=np.float64 if self.type == 'int' else self.dtype).astype(self.dtype)
-            if data.shape == ():
-                data.shape = (1,)
-        return data
+    def size(self, skip=0):
+        """Elements in the fastest axes after dropping the `skip` slowest."""
+        return int(np.prod(self.shape[skip:], dtype=np.int64))
+
+    def _all(self):
+        h = self.header
+        if h.form == "native":
+            a = np.fromfile(h.infile, dtype=h.dtype, count=self.size())
+            if a.size != self.size():
+                raise RsfError("%s: data file holds %d of %d elements" % (self.path, a.size, self.size()))
+            return a.reshape(self.shape)
+        a = np.loadtxt(h.infile, dtype=h.dtype, ndmin=1)
+        return a.reshape(self.shape) if a.size == self.size() else a
+
+    @property
+    def data(self):
+        return self._all()
+
+    def read(self, count=None):
+        """The whole array (count=None), or the next `count` elements of a
+        native file in storage order."""
+        if count is None:
+            return self._all()
+        h = self.header
+        if h.form != "native":
+            raise RsfError("partial reads need a native file")
+        a = np.fromfile(h.infile, dtype=h.dtype, count=count, offset=self._pos * h.dtype.itemsize)
+        self._pos += a.size
+        return a
 
     def traces(self):
-        if self.form == 'native':
-            trc = self.read()
-            while len(trc) > 0:
-                yield trc
-                trc = self.read()
-        else:
-            yield self.read()
-
-    def write(self, data, close=False, fmt='%.18e'):
-        self.write_rsf()
-        if self.form == "native":
-            np.asarray(data).astype(self.dtype).tofile(self.f)
-        elif fmt:
-            np.savetxt(self.f, data, fmt)
-        else:
-            np.savetxt(self.f, data)
-        if close:
-            self.close()
-
-    def read_rsf(self, filename):
-        header = file2str(filename)
-        for key in RSFKEYS_READ:
-            val, found = fromstring(key, header)
-            if found:
-                self.set(**{key: _set_type(key, val)})
-        infile = self.get('in')
-        if not infile:
-            errexit("Error: Cannot find 'in' keyword from rsf header")
-        if not os.path.exists(infile):
-            errexit("Error: File not exists - %s" % infile)
-
-    def write_rsf(self):
-        if self.wrote:
+        """Consecutive traces (n1 samples each), first to last."""
+        if self.header.form != "native":
+            yield from np.atleast_2d(self._all())
             return
-        s = ""
-        for key in RSFKEYS_WRITE:
-            s += "%s=%s\n" % (key, _set_type(key, self.get(key), quote_str=True))
-        for idim in range(self.dimension):
-            s += self.axes[idim].to_str(idim + 1) + "\n"
-        str2file(s, self.filename)
-        self.wrote = True
+        n1 = max(1, self.header.axes[0]["n"])
+        self._pos = 0
+        while True:
+            t = self.read(n1)
+            if t.size == 0:
+                return
+            yield t
 
 
-def input(filename='', mode=None, data_format=None):  # noqa: A001 -- the reference's name
-    r = RSF(mode=mode, readonly=True, data_format=data_format)
-    r.filename = filename
-    r.read_rsf(filename)
-    r.open_in()
-    return r
+class Writer:
+    """An RSF file open for writing (``output``): the header is written
+    with the data."""
+
+    def __init__(self, path, data_format="native_float", **keys):
+        self.path = path
+        infile = keys.pop("in", None) or path + "@"
+        keys.pop("esize", None)   # derived from data_format
+        self.header = Header(os.path.abspath(infile), data_format)
+        for key, val in keys.items():
+            m = _AXIS_KEY.match(key)
+            if not m:
+                raise RsfError("unknown header key %r" % (key,))
+            self.header.axes[int(m.group(2)) - 1][m.group(1)] = _axis_value(m.group(1), val)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+    def close(self):
+        pass
+
+    def write(self, data, fmt=None):
+        h = self.header
+        a = np.asarray(data)
+        if h.ndim == 0:   # no axes given: one axis of all the samples
+            h.axes[0]["n"] = int(a.size)
+        a = a.astype(h.dtype, copy=False)
+        if h.form == "native":
+            np.ascontiguousarray(a).tofile(h.infile)
+        else:
+            if h.type == "complex":
+                raise RsfError("ascii complex data are not supported")
+            n1 = max(1, h.axes[0]["n"])
+            table = a.reshape(-1, n1) if a.ndim > 1 else a.reshape(-1)
+            np.savetxt(h.infile, table, fmt or ("%d" if h.type == "int" else "%.18e"))
+        with open(self.path, "w") as f:
+            f.write(h.text())
 
 
-def output(filename='', mode=None, data_format=None, _in=None, open_in=True, abspath=True, **kwargs):
-    r = RSF(mode=mode, readonly=False, data_format=data_format)
-    r.filename = filename
-    for key in kwargs:
-        _check_key(key)
-    r.set(**kwargs)
-    r.wrote = False
-    if filename and filename.lower() != 'stdout':
-        r.set_in(_in if _in else filename + '@', abspath)
-        if open_in:
-            r.open_in()
-    elif _in:
-        r.set_in(_in, abspath)
-        if open_in:
-            r.open_in()
-    elif open_in:
-        r.open_in(tmpfile=True, abspath=abspath)
-    return r
+def input(filename):  # noqa: A001 -- the reference module's name for it
+    """Open an RSF file for reading."""
+    return Reader(filename)
 
 
-def fromfile(filename, key):
-    """Header values and/or data of an RSF file: ``fromfile(f, "n1 d1 data")``."""
-    with input(filename) as f:
-        if isinstance(key, str):
-            key = key.split()
-        return f.gets(key)
+def output(filename, data_format="native_float", **keys):
+    """An RSF writer for `filename` with header keys (n1=, d1=, label1= ...)."""
+    return Writer(filename, data_format, **keys)
 
 
-def tofile(filename, data, **kwargs):
-    """Write data with the header keys given (n1=..., d1=..., ...)."""
-    with output(filename, **kwargs) as f:
-        f.write(data)
+def fromfile(filename, keys):
+    """Header values and/or the array ("data") of an RSF file.  `keys` is a
+    list or a blank-separated string; one key gives its value, several a
+    list in the same order."""
+    if isinstance(keys, str):
+        keys = keys.split()
+    r = Reader(filename)
+    vals = [r[k] for k in keys]
+    return vals[0] if len(vals) == 1 else vals
+
+
+def tofile(filename, data, **keys):
+    """Write `data` with header keys to `filename` (data in `filename@`)."""
+    Writer(filename, **keys).write(data)
