@@ -59,3 +59,26 @@ def test_chunk_time_block_by_kind_storage_and_size():
     assert chunk_time_block(big, n, 77) == 1           # not a multiple of the block
     assert chunk_time_block(_req("forward", nx=3600, ny=1200, S=12), 12, 0) == 1
     assert chunk_plan(big, 10 ** 10)[1] % 3 == 0
+
+
+def test_admission_fcfs_with_bounded_skip_ahead():
+    from torchwi_amd.serving import select_admissions
+    # the oldest (cost 8) does not fit 6 free bytes: the next two may overtake it
+    assert select_admissions([(0, 8, 0), (0, 3, 0), (0, 3, 0)], 4, 6, 1.0, 4) == ([1, 2], [0, 0])
+    # ... but not once it has been overtaken max_skips times
+    assert select_admissions([(0, 8, 3), (0, 3, 0), (0, 3, 0)], 4, 6, 1.0, 4) == ([1], [0])
+    assert select_admissions([(0, 8, 4), (0, 3, 0)], 4, 6, 1.0, 4) == ([], [])
+    # slots and arrival times bound the admissions; order is kept
+    assert select_admissions([(0, 1, 0), (0, 1, 0), (0, 1, 0)], 2, 10, 1.0, 4) == ([0, 1], [])
+    assert select_admissions([(0, 1, 0), (5, 1, 0)], 4, 10, 1.0, 4) == ([0], [])
+    assert select_admissions([], 4, 10, 1.0, 4) == ([], [])
+
+
+def test_batch_key_separates_what_cannot_share_a_plan():
+    from torchwi_amd.serving import batch_key
+    a, b = _req("gradient"), _req("gradient")
+    assert batch_key(a, 0) == batch_key(b, 0)
+    assert batch_key(a, 0) != batch_key(a, 30)                       # storage mode
+    assert batch_key(a, 0) != batch_key(_req("forward"), 0)          # kind
+    assert batch_key(a, 0) != batch_key(_req("gradient", nt=401), 0)  # time axis
+    assert batch_key(a, 0) != batch_key(_req("gradient", ny=81), 0)   # grid

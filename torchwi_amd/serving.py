@@ -3,16 +3,34 @@ jobs on one GPU.
 
 Requests (a model, its shots, a wavelet, and -- for gradient jobs -- the
 observed gathers) arrive over time.  The server keeps up to ``slots``
-requests on the device at once, each on its own HIP stream so their time
-steps overlap on the GPU, and admits a waiting request as soon as a slot and
-enough of the HBM budget are free (footprint = its largest shot chunk's
-plan plus its inputs and results).  Wavefields, checkpoints, gathers and
-gradients stay device-resident for the whole request.
+requests on the device at once and admits a waiting request as soon as a
+slot and enough of the HBM budget are free (footprint = its largest shot
+chunk's plan plus its inputs and results).  Wavefields, checkpoints,
+gathers and gradients stay device-resident for the whole request.
+
+Execution is in batches: every request whose next chunk of shots is due
+joins a batch with the other due requests of the same kind, grid, order,
+time axis and storage mode; a batch is one shot plan (each request's shots
+on its own model) launched on one of the server's HIP streams, so the shots
+of different requests march in lockstep in the same persistent launches
+(one per time step, or per 3 steps with the temporally blocked kernel,
+replayed from captured hipGraph blocks for one-step plans).  Batches of
+different keys run concurrently on different streams.
 
 A request's shots run in chunks whose size depends only on the request (a
-fixed per-request byte cap), never on what else is running, so each result
-is bit-identical alone and in a mix.  A request whose smallest chunk cannot
-fit the budget is rejected with an error instead of blocking the queue.
+fixed per-request byte cap), and every per-request reduction (the loss, the
+gradient sum over its shots) is taken over that request's own buffers in
+shot order, so each result is bit-identical alone and in a mix.  A request
+whose smallest chunk cannot fit the budget is rejected at submission.  A
+request whose chunk fails to build or launch (bad geometry, allocation
+failure, a library error) is ended with ``error`` set: its slot and budget
+are released and the other requests continue; a failed batch of several
+requests is retried request by request to find the one at fault.
+
+Admission is first come first served with bounded skip-ahead: when the
+oldest waiting request does not fit the free budget, a later one that fits
+may go first, at most ``max_skips`` times, after which nothing overtakes the
+oldest until it fits.
 
 There is no counterpart in the reference, whose shot loop is one process per
 GPU under Horovod (examples/time/inv_torch.py:123-143).
@@ -20,7 +38,7 @@ GPU under Horovod (examples/time/inv_torch.py:123-143).
 import math
 import sys
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import List, Optional
 
 import numpy as np
@@ -77,6 +95,7 @@ class Result:
     grad: Optional[torch.Tensor] = None    # gradient: (nx, ny)
     error: Optional[str] = None
     chunks: int = 0
+    shared_batches: int = 0                # chunks that ran in a batch with other requests
     cell_updates: float = 0.0              # credited work (2 passes for gradients)
 
     @property
@@ -106,10 +125,10 @@ def chunk_plan(req, cap_bytes):
 
 
 def chunk_time_block(req, nshots, K):
-    """Steps per launch for a chunk of nshots shots: gradients by plan size
-    (shots.auto_time_block), forward modelling one step per launch (measured
-    faster at every size on MI355X, e.g. 12 shots of 3608 x 1208: 355 vs 307
-    Gcell-updates/s).  Either gives the same bits."""
+    """Steps per launch for a plan of nshots shots of req's grid: gradients
+    by plan size (shots.auto_time_block), forward modelling one step per
+    launch (measured faster at every size on MI355X, e.g. 12 shots of
+    3608 x 1208: 355 vs 307 Gcell-updates/s).  Either gives the same bits."""
     if req.kind != "gradient" or K % 3:
         return 1
     return auto_time_block(nshots * (req.nx + req.order) * (req.ny + req.order))
@@ -125,25 +144,64 @@ def io_bytes(req):
     return b
 
 
+def batch_key(req, K):
+    """Requests whose chunks can share one plan: same kind, grid, order,
+    time axis and storage mode (each keeps its own model and wavelet)."""
+    return (req.kind, req.nx, req.ny, req.order, req.nt, float(req.h), float(req.dt), K)
+
+
+def select_admissions(waiting, free_slots, free_bytes, now, max_skips):
+    """Indices of waiting entries to admit, in order.  waiting: sequence of
+    (arrival, cost, skips) in arrival order.  First come first served with
+    bounded skip-ahead: an entry that fits may overtake an older one that
+    does not, unless that older one has already been overtaken max_skips
+    times.  Returns (indices, overtaken) where overtaken lists the
+    positions whose skip count grows by one per admitted overtaker."""
+    picked, overtaken = [], []
+    blocked = []   # positions of older entries that did not fit
+    for i, (arrival, cost, skips) in enumerate(waiting):
+        if len(picked) >= free_slots or arrival > now:
+            break
+        if cost <= free_bytes and all(waiting[j][2] + overtaken.count(j) < max_skips for j in blocked):
+            picked.append(i)
+            free_bytes -= cost
+            overtaken.extend(blocked)
+        elif cost > free_bytes:
+            blocked.append(i)
+        else:
+            break  # an older entry has been overtaken too often: wait for it
+    return picked, overtaken
+
+
 class _Active:
-    def __init__(self, req, res, stream, per, K, cost):
-        self.req, self.res, self.stream = req, res, stream
-        self.per, self.K, self.cost = per, K, cost
+    """A request on the device: its accumulators and progress."""
+
+    def __init__(self, req, res, per, K, cost, slot):
+        self.req, self.res = req, res
+        self.per, self.K, self.cost, self.slot = per, K, cost, slot
         self.next_shot = 0
-        self.event = None
+        self.loss = self.grad = self.frd = self.vel = None
+        self.failed = False
+
+
+class _Batch:
+    """One shot plan holding a chunk of each member request."""
+
+    def __init__(self, stream, members):
+        self.stream, self.members = stream, members   # members: [(active, s0, s1)]
         self.plan = None
-        self.loss = None
-        self.grad = None
-        self.frd = None
+        self.event = None
 
 
 class Server:
     """Continuous-batching executor.  budget_bytes: HBM the server may use;
-    slots: requests in flight at once (one HIP stream each, default priority);
-    cap_bytes: per-request chunk cap (default budget/slots)."""
+    slots: requests in flight at once; cap_bytes: per-request chunk cap
+    (default budget/slots); merge: let due chunks of compatible requests
+    share a plan; max_skips: how often a waiting request may be overtaken by
+    later ones that fit."""
 
     def __init__(self, device="cuda", budget_bytes=None, slots=4, cap_bytes=None,
-                 graph_steps=GRAPH_STEPS, verbose=False):
+                 graph_steps=GRAPH_STEPS, verbose=False, merge=True, max_skips=4):
         self.verbose = verbose
         self.device = torch.device(device)
         if budget_bytes is None:
@@ -153,14 +211,20 @@ class Server:
         self.slots = int(slots)
         self.cap = int(cap_bytes) if cap_bytes else self.budget // self.slots
         self.graph_steps = graph_steps
+        self.merge = bool(merge)
+        self.max_skips = int(max_skips)
         self.streams = [torch.cuda.Stream(self.device) for _ in range(self.slots)]
+        self.free_streams = list(range(self.slots))
         self.in_use = 0
         self.free_slots = list(range(self.slots))
-        self.waiting = []
+        self.waiting = []     # [req, res, per, K, cost, skips]
+        self.due = []         # actives whose next chunk should be launched
+        self.batches = []
         self.active = []
         self.done = []
         self.peak_in_use = 0
         self.max_concurrent = 0
+        self.max_batch_requests = 0
 
     # ---- queue ------------------------------------------------------------
     def submit(self, req):
@@ -175,102 +239,201 @@ class Server:
             if cost > self.budget:
                 res.error = "request needs %d bytes of HBM, budget is %d" % (cost, self.budget)
             else:
-                self.waiting.append((req, res, per, K, cost))
+                self.waiting.append([req, res, per, K, cost, 0])
                 return res
         res.finished = res.admitted = time.perf_counter()
         self.done.append(res)
         return res
 
-    # ---- execution ----------------------------------------------------------
+    # ---- admission ----------------------------------------------------------
     def _admit(self, now):
-        while self.waiting and self.free_slots:
-            req, res, per, K, cost = self.waiting[0]
-            if req.arrival > now:
-                break
-            if self.in_use + cost > self.budget:
-                break  # FIFO: wait for memory instead of starving large jobs
-            self.waiting.pop(0)
+        view = [(w[0].arrival, w[4], w[5]) for w in self.waiting]
+        picked, overtaken = select_admissions(view, len(self.free_slots), self.budget - self.in_use, now,
+                                              self.max_skips)
+        for j in overtaken:
+            self.waiting[j][5] += 1
+        for i in picked:
+            req, res, per, K, cost, _ = self.waiting[i]
             slot = self.free_slots.pop(0)
             self.in_use += cost
             self.peak_in_use = max(self.peak_in_use, self.in_use)
             res.admitted = time.perf_counter()
-            a = _Active(req, res, self.streams[slot], per, K, cost)
-            a.slot = slot
+            a = _Active(req, res, per, K, cost, slot)
             self.active.append(a)
             self.max_concurrent = max(self.max_concurrent, len(self.active))
-            with torch.cuda.stream(a.stream):
+            try:
                 dev = self.device
-                a.vel = req.vel.to(dev, torch.float32, non_blocking=False)
+                a.vel = req.vel.to(dev, torch.float32)
                 if req.kind == "gradient":
                     a.loss = torch.zeros((), dtype=torch.float64, device=dev)
                     a.grad = torch.zeros(req.ny + req.order, req.nx + req.order, dtype=torch.float32, device=dev)
                 else:
                     a.frd = torch.empty(req.nshots, req.nt, req.nx, dtype=torch.float32, device=dev)
-            self._launch_chunk(a)
+            except Exception as e:  # noqa: BLE001 -- any failure ends this request only
+                self._fail(a, e)
+                continue
+            self.due.append(a)
+        for i in sorted(picked, reverse=True):
+            self.waiting.pop(i)
 
-    def _launch_chunk(self, a):
-        req = a.req
-        s0 = a.next_shot
-        s1 = min(req.nshots, s0 + a.per)
-        shots = [dict(model=0, sx=req.sx[i], sy=req.sy[i], ry=req.ry[i], rx0=0, nrec=req.nx, w=req.w)
-                 for i in range(s0, s1)]
-        with torch.cuda.stream(a.stream):
-            a.plan = None
-            p = ShotPlan(req.order, req.nt, req.h, req.dt, [(req.nx, req.ny)], shots, a.K, self.device,
-                         graph_steps=self.graph_steps, forward_only=req.kind == "forward",
-                         time_block=chunk_time_block(req, s1 - s0, a.K))
-            p.models[0].load(a.vel)
-            p.forward()
-            if req.kind == "gradient":
-                r = p.resid_all()
-                torch.sub(p.frd_all(), req.d_obs[s0:s1], out=r)
-                rv = r.view(-1)
-                a.loss += 0.5 * torch.dot(rv, rv).double()
-                p.backward()
-                for s in range(p.S):
-                    a.grad += p.grad_of(s)[:, :req.nx + req.order]
-            else:
-                a.frd[s0:s1].copy_(p.frd_all())
-            a.plan = p
-            a.event = torch.cuda.Event()
-            a.event.record(a.stream)
-        a.res.chunks += 1
-        cells = (req.nx + req.order) * (req.ny + req.order) * req.nt * (s1 - s0)
-        a.res.cell_updates += cells * (2 if req.kind == "gradient" else 1)
-        a.next_shot = s1
+    # ---- execution ----------------------------------------------------------
+    def _fail(self, a, err):
+        if a.failed:
+            return
+        a.failed = True
+        a.res.error = "%s: %s" % (type(err).__name__, err)
+        a.res.finished = time.perf_counter()
+        if self.verbose:
+            print("serving: request %d failed: %s" % (a.req.rid, a.res.error), file=sys.stderr, flush=True)
+        self._release(a)
+
+    def _release(self, a):
+        self.in_use -= a.cost
+        self.free_slots.append(a.slot)
+        self.free_slots.sort()
+        self.active = [x for x in self.active if x is not a]
+        a.loss = a.grad = a.frd = a.vel = None
+        self.done.append(a.res)
+
+    def _run_batch(self, members):
+        """Build and launch one plan over the members' next chunks (on a free
+        stream); results are accumulated per member on that stream."""
+        sid = self.free_streams.pop(0)
+        b = _Batch(sid, members)
+        stream = self.streams[sid]
+        try:
+            req0 = members[0][0].req
+            models, shots = [], []
+            for i, (a, s0, s1) in enumerate(members):
+                r = a.req
+                models.append((r.nx, r.ny))
+                shots += [dict(model=i, sx=r.sx[k], sy=r.sy[k], ry=r.ry[k], rx0=0, nrec=r.nx, w=r.w)
+                          for k in range(s0, s1)]
+            K = members[0][0].K
+            with torch.cuda.stream(stream):
+                p = ShotPlan(req0.order, req0.nt, req0.h, req0.dt, models, shots, K, self.device,
+                             graph_steps=self.graph_steps, forward_only=req0.kind == "forward",
+                             time_block=chunk_time_block(req0, len(shots), K))
+                for i, (a, _, _) in enumerate(members):
+                    p.models[i].load(a.vel)
+                p.forward()
+                o = 0
+                if req0.kind == "gradient":
+                    rall = p.resid_all()
+                    for a, s0, s1 in members:
+                        torch.sub(p.frd_all()[o:o + s1 - s0], a.req.d_obs[s0:s1], out=rall[o:o + s1 - s0])
+                        o += s1 - s0
+                    p.backward()
+        except Exception:
+            self.free_streams.append(sid)
+            self.free_streams.sort()
+            raise
+        # Accumulate into the members' results.  A failure past this point
+        # would leave partial sums, so it ends every member instead of being
+        # retried.
+        try:
+            with torch.cuda.stream(stream):
+                o = 0
+                for a, s0, s1 in members:
+                    n = s1 - s0
+                    if req0.kind == "gradient":
+                        # the request's own residual rows, in a buffer of their own
+                        rv = rall[o:o + n].clone().view(-1)
+                        a.loss += 0.5 * torch.dot(rv, rv).double()
+                        for s in range(o, o + n):
+                            a.grad += p.grad_of(s)[:, :a.req.nx + a.req.order]
+                    else:
+                        a.frd[s0:s1].copy_(p.frd_all()[o:o + n])
+                    o += n
+                b.plan = p
+                b.event = torch.cuda.Event()
+                b.event.record(stream)
+        except Exception as e:  # noqa: BLE001
+            self.free_streams.append(sid)
+            self.free_streams.sort()
+            for a, _, _ in members:
+                self._fail(a, e)
+            return
+        for a, s0, s1 in members:
+            a.res.chunks += 1
+            if len(members) > 1:
+                a.res.shared_batches += 1
+            r = a.req
+            cells = (r.nx + r.order) * (r.ny + r.order) * r.nt * (s1 - s0)
+            a.res.cell_updates += cells * (2 if r.kind == "gradient" else 1)
+            a.next_shot = s1
+        self.max_batch_requests = max(self.max_batch_requests, len(members))
+        self.batches.append(b)
+
+    def _launch_due(self):
+        groups = {}
+        for a in self.due:
+            if not a.failed:
+                groups.setdefault(batch_key(a.req, a.K) if self.merge else id(a), []).append(a)
+        self.due = []
+        for group in groups.values():
+            members = [(a, a.next_shot, min(a.req.nshots, a.next_shot + a.per)) for a in group]
+            try:
+                self._run_batch(members)
+            except Exception as e:  # noqa: BLE001
+                if len(members) == 1:
+                    self._fail(members[0][0], e)
+                    continue
+                # find the request at fault: run each member's chunk alone
+                for m in members:
+                    try:
+                        self._run_batch([m])
+                    except Exception as e1:  # noqa: BLE001
+                        self._fail(m[0], e1)
 
     def _retire(self):
         still = []
-        for a in self.active:
-            if not a.event.query():
-                still.append(a)
+        for b in self.batches:
+            try:
+                if not b.event.query():
+                    still.append(b)
+                    continue
+            except Exception as e:  # noqa: BLE001
+                for a, _, _ in b.members:
+                    self._fail(a, e)
+                b.plan = None
+                self.free_streams.append(b.stream)
                 continue
-            req = a.req
-            if a.next_shot < req.nshots:
-                self._launch_chunk(a)
-                still.append(a)
-                continue
-            a.plan = None
-            ne = req.order // 2
-            if req.kind == "gradient":
-                a.res.loss = float(a.loss)
-                a.res.grad = a.grad[ne:ne + req.ny, ne:ne + req.nx].transpose(0, 1)
-            else:
-                a.res.frd = a.frd
-            a.res.finished = time.perf_counter()
-            if self.verbose:
-                print("serving: request %d (%s) done, latency %.1f ms" % (req.rid, req.kind, 1e3 * a.res.latency),
-                      file=sys.stderr, flush=True)
-            self.in_use -= a.cost
-            self.free_slots.append(a.slot)
-            self.free_slots.sort()
-            self.done.append(a.res)
-        self.active = still
+            b.plan = None
+            self.free_streams.append(b.stream)
+            for a, _, _ in b.members:
+                if a.failed:
+                    continue
+                if a.next_shot < a.req.nshots:
+                    self.due.append(a)
+                else:
+                    self._finish(a)
+        self.free_streams.sort()
+        self.batches = still
+
+    def _finish(self, a):
+        req = a.req
+        ne = req.order // 2
+        if req.kind == "gradient":
+            a.res.loss = float(a.loss)
+            a.res.grad = a.grad[ne:ne + req.ny, ne:ne + req.nx].transpose(0, 1)
+        else:
+            a.res.frd = a.frd
+        a.res.finished = time.perf_counter()
+        if self.verbose:
+            print("serving: request %d (%s) done, latency %.1f ms" % (req.rid, req.kind, 1e3 * a.res.latency),
+                  file=sys.stderr, flush=True)
+        self.in_use -= a.cost
+        self.free_slots.append(a.slot)
+        self.free_slots.sort()
+        self.active = [x for x in self.active if x is not a]
+        self.done.append(a.res)
 
     def poll(self, now=None):
         now = time.perf_counter() if now is None else now
         self._retire()
         self._admit(now)
+        self._launch_due()
 
     def idle(self):
         return not self.waiting and not self.active
@@ -302,16 +465,35 @@ class Server:
 GRIDS = ((256, 512), (512, 1024), (800, 2400), (1200, 3600))  # (nz, nx)
 
 
+def make_request(kind, nz, nx, nt, nshots, freq, seed, h=10.0, dt=1e-3, order=8, device="cuda",
+                 arrival=0.0, rid=-1):
+    """One request of the serving stream: a random layered + correlated
+    perturbation model in 1500-4500 m/s (seeded), nshots ricker(freq) shots
+    evenly spaced at z=2 with receivers on every column; gradient requests
+    get a seeded random field of the gathers' shape as observed data
+    (modelling a perturbed model would double the work)."""
+    from .source.wavelets import ricker
+    from .workloads import dipping_layers, to_user, cell_pos
+    rng = np.random.default_rng(seed)
+    v = dipping_layers(nz, nx, seed=int(rng.integers(1 << 30)), perturb=0.05)
+    vel = torch.from_numpy(to_user(v))
+    w = torch.from_numpy(ricker(freq, dt, nt).astype(np.float32))
+    xs = (np.arange(nshots) + 0.5) * nx / nshots
+    sx = [float(cell_pos(int(x), h)) for x in xs]
+    d_obs = None
+    if kind == "gradient":
+        g = torch.Generator(device="cpu").manual_seed(int(rng.integers(1 << 30)))
+        d_obs = (1e-3 * torch.randn(nshots, nt, nx, generator=g)).to(device)
+    return Request(kind=kind, vel=vel, w=w, sx=sx, sy=[float(cell_pos(2, h))] * nshots,
+                   ry=[float(cell_pos(2, h))] * nshots, h=h, dt=dt, order=order, d_obs=d_obs,
+                   arrival=arrival, rid=rid)
+
+
 def make_stream(n, rate_per_s, seed=5, grids=GRIDS, nt_range=(2000, 6000), shots_range=(1, 64),
                 freq_range=(5.0, 15.0), h=10.0, dt=1e-3, order=8, device="cuda", frac_gradient=0.5):
     """n requests with Poisson arrivals (seeded): grid from `grids`, nt uniform
-    in nt_range, shots uniform in shots_range at z=2 with receivers on every
-    column, ricker(freq uniform in freq_range), random layered + correlated
-    perturbation model in 1500-4500 m/s, half forward-only and half gradient
-    (observed data: the gathers of a 2%-perturbed model would double the
-    work, so a seeded random field of the gathers' shape stands in)."""
-    from .source.wavelets import ricker
-    from .workloads import dipping_layers, to_user, cell_pos
+    in nt_range, shots uniform in shots_range, ricker(freq uniform in
+    freq_range), half forward-only and half gradient (make_request)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     reqs = []
@@ -322,18 +504,8 @@ def make_stream(n, rate_per_s, seed=5, grids=GRIDS, nt_range=(2000, 6000), shots
         S = int(rng.integers(shots_range[0], shots_range[1] + 1))
         f = float(rng.uniform(*freq_range))
         kind = "gradient" if rng.random() < frac_gradient else "forward"
-        v = dipping_layers(nz, nx, seed=int(rng.integers(1 << 30)), perturb=0.05)
-        vel = torch.from_numpy(to_user(v))
-        w = torch.from_numpy(ricker(f, dt, nt).astype(np.float32))
-        xs = (np.arange(S) + 0.5) * nx / S
-        sx = [float(cell_pos(int(x), h)) for x in xs]
-        d_obs = None
-        if kind == "gradient":
-            g = torch.Generator(device="cpu").manual_seed(int(rng.integers(1 << 30)))
-            d_obs = (1e-3 * torch.randn(S, nt, nx, generator=g)).to(device)
-        reqs.append(Request(kind=kind, vel=vel, w=w, sx=sx, sy=[float(cell_pos(2, h))] * S,
-                            ry=[float(cell_pos(2, h))] * S, h=h, dt=dt, order=order, d_obs=d_obs,
-                            arrival=t, rid=k))
+        reqs.append(make_request(kind, nz, nx, nt, S, f, int(rng.integers(1 << 30)), h=h, dt=dt, order=order,
+                                 device=device, arrival=t, rid=k))
     return reqs
 
 
@@ -347,6 +519,7 @@ def summarize(results, wall_s):
         "mean_ms": round(float(np.mean(lat)), 2), "wall_s": round(wall_s, 3),
         "throughput_req_per_s": round(len(ok) / wall_s, 4),
         "throughput_Gcell_updates_per_s": round(cells / wall_s / 1e9, 2),
+        "chunks_in_shared_batches": int(sum(r.shared_batches for r in results)),
     }
 
 
@@ -360,15 +533,17 @@ def main(argv=None):
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--scale", type=float, default=1.0, help="scale grids and nt (tests use < 1)")
+    ap.add_argument("--no-merge", action="store_true", help="one plan per request chunk")
     args = ap.parse_args(argv)
     sc = args.scale
     grids = tuple((max(32, int(a * sc)), max(64, int(b * sc))) for a, b in GRIDS)
     ntr = (max(50, int(2000 * sc)), max(60, int(6000 * sc)))
     shr = (1, max(1, int(64 * sc)))
     dev = torch.device("cuda:0")
+    merge = not args.no_merge
     # capacity probe: serve the stream back to back (all arrivals at t=0)
     probe = make_stream(args.requests, 1.0, seed=args.seed, grids=grids, nt_range=ntr, shots_range=shr)
-    srv = Server(dev, slots=args.slots, verbose=True)
+    srv = Server(dev, slots=args.slots, verbose=True, merge=merge)
     t0 = time.perf_counter()
     srv.run(probe, realtime=False)
     cap_s = time.perf_counter() - t0
@@ -377,15 +552,16 @@ def main(argv=None):
     torch.cuda.empty_cache()
     rate = args.rate or 0.8 * capacity
     reqs = make_stream(args.requests, rate, seed=args.seed, grids=grids, nt_range=ntr, shots_range=shr)
-    srv = Server(dev, slots=args.slots, verbose=True)
+    srv = Server(dev, slots=args.slots, verbose=True, merge=merge)
     t0 = time.perf_counter()
     res = srv.run(reqs, realtime=True)
     wall = time.perf_counter() - t0
     out = {"workload": "serving stream: %d Poisson requests (seed %d) at %.3f req/s (80%% of the %.3f req/s "
                        "back-to-back capacity), grids %s, nt %s, shots %s, 50/50 forward/gradient"
                        % (args.requests, args.seed, rate, capacity, list(grids), ntr, shr),
-           "slots": args.slots, "budget_GB": round(srv.budget / 1e9, 1),
-           "peak_in_use_GB": round(srv.peak_in_use / 1e9, 2), "max_concurrent": srv.max_concurrent}
+           "slots": args.slots, "merge": merge, "budget_GB": round(srv.budget / 1e9, 1),
+           "peak_in_use_GB": round(srv.peak_in_use / 1e9, 2), "max_concurrent": srv.max_concurrent,
+           "max_requests_per_batch": srv.max_batch_requests}
     out.update(summarize(res, wall))
     print(json.dumps(out), flush=True)
 
