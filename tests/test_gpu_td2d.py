@@ -4,14 +4,14 @@ The HIP kernels reproduce the reference CPU path's mixed-precision
 arithmetic, so the bar here is bit-exact equality with the oracle (which is
 itself bit-exact with the compiled reference, tests/test_oracle.py).  The
 parity tolerance for floating point, relative L2 <= 1e-5, is asserted as well, and the FP32
-error against the FP64 run of the same recursion is printed per result.
+error against the FP64 run of the same recursion is recorded per result (tests.helpers.record_drift).
 """
 import numpy as np
 import pytest
 import torch
 
 from oracle import oracle as O
-from tests.helpers import rel_l2, small_case
+from tests.helpers import record_drift, rel_l2, small_case
 
 pytestmark = pytest.mark.gpu
 
@@ -54,8 +54,9 @@ def test_forward_backward_bitexact(order, shape):
     got = _run_module_forward_backward(c, resid)
     frd64, virt64 = O.forward64(vpad, c["w"], *args, c["sx"], c["sy"], c["ry"])
     grad64 = O.backward64(virt64, vpad, resid, *args, c["ry"])
-    print(f"  rel-L2 vs oracle f32: frd {rel_l2(got['frd'], frd_o):.3e} grad {rel_l2(got['grad'], grad_o):.3e}; "
-          f"vs f64: frd {rel_l2(got['frd'], frd64):.3e} grad {rel_l2(got['grad'], grad64):.3e}")
+    print(f"  rel-L2 vs oracle f32: frd {rel_l2(got['frd'], frd_o):.3e} grad {rel_l2(got['grad'], grad_o):.3e}")
+    record_drift(f"small order={order} nx={nx} ny={ny} nt=160", frd=rel_l2(got["frd"], frd64),
+                 grad=rel_l2(got["grad"], grad64))
     assert rel_l2(got["frd"], frd_o) <= TOL and rel_l2(got["grad"], grad_o) <= TOL
     assert np.array_equal(got["frd"], frd_o)
     assert np.array_equal(got["virt"], virt_o)
@@ -204,8 +205,9 @@ def test_three_layer_100x200_forward_gradient_bitexact():
     loss = 0.5 * torch.sum((frd - torch.from_numpy(d_obs).to(DEV)) ** 2)
     loss.backward()
     f, g = frd.detach().cpu().numpy(), vel.grad.cpu().numpy()
-    print(f"three-layer 100x200: rel-L2 vs oracle frd {rel_l2(f, frd_o):.2e} grad {rel_l2(g, grad_o):.2e}; "
-          f"vs FP64 frd {rel_l2(f, frd64):.2e} grad {rel_l2(g, grad64):.2e}")
+    print(f"three-layer 100x200: rel-L2 vs oracle frd {rel_l2(f, frd_o):.2e} grad {rel_l2(g, grad_o):.2e}")
+    record_drift("config 1: three-layer 100x200 + 20-cell pad, order 8, nt=1000", frd=rel_l2(f, frd64),
+                 grad=rel_l2(g, grad64))
     assert np.array_equal(f, frd_o)
     assert np.array_equal(g, grad_o)
 
