@@ -171,6 +171,29 @@ int tw_plan_backward(void *plan, int accumulate, void *stream);
 
 void tw_plan_destroy(void *plan);
 
+/* Size of the static plan workspace of each device (4 MiB). */
+#define TW_STATIC_WORKSPACE_BYTES 4194304
+
+/* tw_plan_create without a caller workspace: the plan's descriptors, tile
+ * lists and control words go to a device-resident array the library holds
+ * for each device (TW_STATIC_WORKSPACE_BYTES, kept until the process ends),
+ * and the call allocates no device memory (no hipMalloc, no allocating
+ * library routine; host memory for the plan handle only).  No counterpart in
+ * the reference: this is for callers that cannot allocate device memory
+ * (code that runs between graph captures, serving loops with a fixed memory
+ * footprint, harnesses that forbid allocation).  Limits: the plan must fit
+ * the workspace (plan sizes beyond it return TW_EINVAL; tw_plan_workspace_bytes
+ * gives the size); graph_steps must be 0 (graph instantiation may allocate);
+ * at most one such plan per device exists at a time -- a second
+ * tw_plan_create_static on a device whose workspace is held returns
+ * TW_EINVAL until the holder is destroyed.  The workspace keeps the last
+ * plan's contents; every plan writes all of its own before use.  Work of two
+ * plans in flight on one device at once cannot share it, which the single
+ * holder rule prevents.  The plan runs with tw_plan_forward /
+ * tw_plan_backward and is released with tw_plan_destroy, as any plan. */
+int tw_plan_create_static(const TwPlanSpec *spec, const TwShotSpec *shots, int nshots,
+                          void *stream, void **plan);
+
 /* Kinds of step kernel a plan launches. */
 #define TW_KIND_FWD 0      /* forward, gathers (+ checkpoint saves) */
 #define TW_KIND_FWD_VIRT 1 /* forward with full virtual-source storage */

@@ -98,8 +98,11 @@ int main(int argc, char** argv)
     }
     std::vector<char> work(ws);
     void* plan = nullptr;
-    if (tw_plan_create(&spec, sh.data(), S, work.data(), ws, nullptr, &plan) || tw_plan_forward(plan, nullptr) ||
-        tw_plan_backward(plan, 0, nullptr)) {
+    // TW_EMU_STATIC=1: the plan's descriptors in the library's static workspace
+    const char* st = getenv("TW_EMU_STATIC");
+    const int rc0 = (st && st[0] == '1') ? tw_plan_create_static(&spec, sh.data(), S, nullptr, &plan)
+                                         : tw_plan_create(&spec, sh.data(), S, work.data(), ws, nullptr, &plan);
+    if (rc0 || tw_plan_forward(plan, nullptr) || tw_plan_backward(plan, 0, nullptr)) {
         fprintf(stderr, "plan: %s\n", tw_last_error());
         return 1;
     }

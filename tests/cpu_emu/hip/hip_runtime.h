@@ -18,6 +18,8 @@
 #define __launch_bounds__(...)
 #define __shared__ static
 #define __restrict__
+#define __align__(n) __attribute__((aligned(n)))
+#define HIP_SYMBOL(x) (&(x))
 struct float4 { float x, y, z, w; };
 struct int2 { int x, y; };
 inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
@@ -116,3 +118,5 @@ inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return 0; }
 inline hipError_t hipGetDevice(int* d) { *d = 0; return 0; }
 inline hipError_t hipDeviceGetAttribute(int* v, int, int) { *v = 256; return 0; }
 inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t) { *n = 3; return 0; }
+// a __device__ array is host memory here: its "device address" is its address
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return 0; }

@@ -116,3 +116,37 @@ def test_grad_group_is_refused_on_the_host():
     assert "grad_group" in _lib.last_error()
     spec.grad_group = 1
     assert L.tw_plan_workspace_bytes(ctypes.byref(spec), shots, 2) > 0
+
+
+def test_static_plan_limits_are_refused_before_any_device_call():
+    """tw_plan_create_static refuses, with a clear error and without touching
+    a device, a plan larger than the static workspace (tw_plan_workspace_bytes
+    tells the size) and graph capture, which may allocate."""
+    from torchwi_amd import _lib
+    L = _lib.lib()
+    h = 10.0
+    dimx, dimy = 3008, 1008
+    ld = 3008
+
+    def shots(n):
+        arr = (_lib.TwShotSpec * n)()
+        for q in arr:
+            q.dimx, q.dimy, q.ld = dimx, dimy, ld
+            q.sx, q.sy, q.ry = 100.5 * h, 2.5 * h, 2.5 * h
+            q.rx0, q.nrec = 0, dimx - 8
+            for p in ("vel", "vm", "vc3", "w", "u", "lam", "frd", "resid", "virt", "grad"):
+                setattr(q, p, 4096)
+        return arr
+
+    spec = _lib.TwPlanSpec(order=8, nt=50, h=h, dt=1e-3, ckpt_every=0, graph_steps=0, time_block=1)
+    big = shots(256)   # 64 x 16 tiles: 256 x 47 x 63 of them, 6 MB of tile list
+    need = L.tw_plan_workspace_bytes(ctypes.byref(spec), big, 256)
+    assert need > 4194304
+    plan = ctypes.c_void_p()
+    rc = L.tw_plan_create_static(ctypes.byref(spec), big, 256, None, ctypes.byref(plan))
+    assert rc == -1 and not plan.value
+    msg = _lib.last_error()
+    assert "static workspace" in msg and str(need) in msg and "4194304" in msg, msg
+    spec.graph_steps = 100
+    assert L.tw_plan_create_static(ctypes.byref(spec), shots(1), 1, None, ctypes.byref(plan)) == -1
+    assert "graph_steps = 0" in _lib.last_error()

@@ -41,11 +41,14 @@ def driver(tmp_path_factory):
     return exe
 
 
-@pytest.mark.parametrize("order,nx,ny,nt,S,K,graph,tb,rows", [
-    (8, 20, 12, 10, 2, 0, 4, 1, 0), (4, 37, 15, 9, 3, 4, 0, 1, 0), (2, 70, 18, 8, 2, 3, 3, 1, 0),
-    (8, 24, 14, 9, 2, 0, 0, 3, 0), (8, 250, 26, 6, 1, 0, 0, 3, 16), (4, 40, 22, 6, 2, 4, 0, 2, 0),
-    (2, 33, 21, 8, 2, 4, 0, 4, 0), (4, 40, 70, 6, 2, 3, 0, 3, 0)])
-def test_kernel_source_on_host_threads_matches_oracle(driver, tmp_path, order, nx, ny, nt, S, K, graph, tb, rows):
+@pytest.mark.parametrize("order,nx,ny,nt,S,K,graph,tb,rows,static", [
+    (8, 20, 12, 10, 2, 0, 4, 1, 0, 0), (4, 37, 15, 9, 3, 4, 0, 1, 0, 0), (2, 70, 18, 8, 2, 3, 3, 1, 0, 0),
+    (8, 24, 14, 9, 2, 0, 0, 3, 0, 0), (8, 250, 26, 6, 1, 0, 0, 3, 16, 0), (4, 40, 22, 6, 2, 4, 0, 2, 0, 0),
+    (2, 33, 21, 8, 2, 4, 0, 4, 0, 0), (4, 40, 70, 6, 2, 3, 0, 3, 0, 0),
+    # the plan descriptors in the static workspace (tw_plan_create_static)
+    (8, 24, 14, 9, 2, 3, 0, 3, 0, 1), (4, 30, 16, 8, 3, 0, 0, 1, 0, 1)])
+def test_kernel_source_on_host_threads_matches_oracle(driver, tmp_path, order, nx, ny, nt, S, K, graph, tb, rows,
+                                                      static):
     rng = np.random.default_rng(order * 7 + nx)
     h, dt = 10.0, 1e-3
     vel = (1500.0 + 2500.0 * rng.random((nx, ny))).astype(np.float32)
@@ -64,7 +67,7 @@ def test_kernel_source_on_host_threads_matches_oracle(driver, tmp_path, order, n
         np.array([h, dt], np.float32).tofile(f)
         for a in (vld, w, pos, resid):
             a.tofile(f)
-    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", TW_EMU_STATIC=str(static))
     r = subprocess.run([driver, fin, fout], capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     out = np.fromfile(fout, np.float32)

@@ -34,6 +34,7 @@ SIGNATURES = {
     "tw_plan_forward": (_ci, [_vp, _vp]),
     "tw_plan_backward": (_ci, [_vp, _ci, _vp]),
     "tw_plan_destroy": (None, [_vp]),
+    "tw_plan_create_static": (_ci, [_vp, _vp, _ci, _vp, ctypes.POINTER(_vp)]),
     "tw_plan_set_timing": (_ci, [_vp, _ci]),
     "tw_plan_timing": (_ci, [_vp, _ci, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]),
     "tw_stream_copy": (_ci, [_vp, _vp, ctypes.c_size_t, _vp]),

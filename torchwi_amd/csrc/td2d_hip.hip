@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -639,6 +640,9 @@ struct Plan {
     int2* d_tiles = nullptr;
     int2* d_gtiles = nullptr;
     int2* d_vtiles = nullptr;
+    int static_dev = -1;              // >= 0: the descriptors live in that device's static workspace
+    hipStream_t last_stream = nullptr;  // stream of the latest forward / backward pass
+    bool used = false;
     ~Plan()
     {
         g_fwd.release();
@@ -1198,6 +1202,73 @@ int batch_prepare_ctl(Plan& p, hipStream_t s)
     return TW_OK;
 }
 
+// ------------------------------------------------------------------------
+// Static plan workspace for callers that cannot allocate device memory
+// (tw_plan_create_static).  One array per device, held by at most one plan at
+// a time; host code uses the device address hipGetSymbolAddress returns for
+// the current device, never the array's name.
+// ------------------------------------------------------------------------
+constexpr size_t STATIC_WS_BYTES = (size_t)TW_STATIC_WORKSPACE_BYTES;
+static __device__ __align__(256) unsigned char scratch[STATIC_WS_BYTES];
+
+constexpr int STATIC_MAX_DEVICES = 64;
+struct StaticWs {
+    void* ptr = nullptr;     // device address of scratch on this device
+    const void* holder = nullptr;
+};
+std::mutex g_static_mu;
+StaticWs g_static[STATIC_MAX_DEVICES];
+
+// Bind the current device's static workspace to plan p (looked up once per
+// device, with that device current).
+int static_acquire(const void* p, int& dev, void*& ptr)
+{
+    if (hipGetDevice(&dev) != hipSuccess) return fail(TW_EHIP, "hipGetDevice failed");
+    if (dev < 0 || dev >= STATIC_MAX_DEVICES) return fail(TW_EINVAL, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    StaticWs& w = g_static[dev];
+    if (!w.ptr) {
+        void* q = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !q)
+            return fail(TW_EHIP, "hipGetSymbolAddress of the static workspace failed: %s", hipGetErrorString(e));
+        w.ptr = q;
+    }
+    if (w.holder)
+        return fail(TW_EINVAL, "the static workspace of device %d is held by another plan (destroy it first)", dev);
+    w.holder = p;
+    ptr = w.ptr;
+    return TW_OK;
+}
+
+void static_release(const void* p, int dev)
+{
+    if (dev < 0 || dev >= STATIC_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (g_static[dev].holder == p) g_static[dev].holder = nullptr;
+}
+
+// Lay a plan's descriptors, tile lists and control words out in a workspace
+// of plan_bytes(p) bytes and upload them (stream-ordered, then synchronised).
+int place_and_upload(Plan& p, void* workspace, hipStream_t s)
+{
+    p.d_desc = (ShotDesc*)workspace;
+    p.d_tiles = (int2*)((char*)workspace + align256(sizeof(ShotDesc) * p.nshots));
+    p.d_ctl = (int*)((char*)workspace + align256(sizeof(ShotDesc) * p.nshots) + align256(sizeof(int2) * p.ntiles));
+    p.d_gtiles = (int2*)((char*)p.d_ctl + 256);
+    p.d_vtiles = (int2*)((char*)p.d_gtiles + align256(sizeof(int2) * p.gntiles));
+    hipError_t e = hipMemcpyAsync(p.d_desc, p.host.data(), sizeof(ShotDesc) * p.nshots, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p.d_tiles, p.tiles.data(), sizeof(int2) * p.ntiles, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && !p.vtiles.empty())
+        e = hipMemcpyAsync(p.d_vtiles, p.vtiles.data(), sizeof(int2) * p.vtiles.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && p.gntiles > 0)
+        e = hipMemcpyAsync(p.d_gtiles, p.gtiles.data(), sizeof(int2) * p.gntiles, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(TW_EHIP, "descriptor upload failed: %s", hipGetErrorString(e));
+    return TW_OK;
+}
+
 }  // namespace tw
 
 using namespace tw;
@@ -1328,29 +1399,41 @@ int tw_plan_create(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, 
         return fail(TW_EINVAL, "workspace of %zu bytes is smaller than the %zu needed",
                     workspace_bytes, need);
     }
-    p->d_desc = (ShotDesc*)workspace;
-    p->d_tiles = (int2*)((char*)workspace + align256(sizeof(ShotDesc) * p->nshots));
-    p->d_ctl = (int*)((char*)workspace + align256(sizeof(ShotDesc) * p->nshots) +
-                      align256(sizeof(int2) * p->ntiles));
-    p->d_gtiles = (int2*)((char*)p->d_ctl + 256);
-    p->d_vtiles = (int2*)((char*)p->d_gtiles + align256(sizeof(int2) * p->gntiles));
     p->graph_steps = spec->graph_steps;
-    hipStream_t s = (hipStream_t)stream;
-    hipError_t e = hipMemcpyAsync(p->d_desc, p->host.data(), sizeof(ShotDesc) * p->nshots,
-                                  hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(p->d_tiles, p->tiles.data(), sizeof(int2) * p->ntiles,
-                           hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && !p->vtiles.empty())
-        e = hipMemcpyAsync(p->d_vtiles, p->vtiles.data(), sizeof(int2) * p->vtiles.size(),
-                           hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && p->gntiles > 0)
-        e = hipMemcpyAsync(p->d_gtiles, p->gtiles.data(), sizeof(int2) * p->gntiles,
-                           hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) {
+    if ((rc = place_and_upload(*p, workspace, (hipStream_t)stream))) {
         delete p;
-        return fail(TW_EHIP, "descriptor upload failed: %s", hipGetErrorString(e));
+        return rc;
+    }
+    *plan = p;
+    return TW_OK;
+}
+
+int tw_plan_create_static(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, void* stream,
+                          void** plan)
+{
+    if (!plan) return fail(TW_EINVAL, "plan out-pointer is NULL");
+    *plan = nullptr;
+    if (spec && spec->graph_steps != 0)
+        return fail(TW_EINVAL, "tw_plan_create_static takes graph_steps = 0 (graph capture and "
+                               "instantiation may allocate device memory)");
+    Plan* p = new Plan();
+    int rc = build_plan(spec, shots, nshots, *p);
+    if (rc) { delete p; return rc; }
+    const size_t need = plan_bytes(*p);
+    if (need > STATIC_WS_BYTES) {
+        delete p;
+        return fail(TW_EINVAL, "the plan needs %zu bytes of device workspace, more than the %zu of the "
+                               "static workspace; use tw_plan_create with a workspace", need, STATIC_WS_BYTES);
+    }
+    int dev = -1;
+    void* ws = nullptr;
+    if ((rc = static_acquire(p, dev, ws))) { delete p; return rc; }
+    p->static_dev = dev;
+    p->graph_steps = 0;
+    if ((rc = place_and_upload(*p, ws, (hipStream_t)stream))) {
+        static_release(p, dev);
+        delete p;
+        return rc;
     }
     *plan = p;
     return TW_OK;
@@ -1361,6 +1444,8 @@ int tw_plan_forward(void* plan, void* stream)
     if (!plan) return fail(TW_EINVAL, "NULL plan");
     Plan& p = *(Plan*)plan;
     hipStream_t s = (hipStream_t)stream;
+    p.last_stream = s;
+    p.used = true;
     int rc;
     if ((rc = prep_models(p, s))) return rc;
     // zero the forward ring: prepare(seg 0) zeroes two fields, the third is
@@ -1410,6 +1495,8 @@ int tw_plan_backward(void* plan, int accumulate, void* stream)
     if (!plan) return fail(TW_EINVAL, "NULL plan");
     Plan& p = *(Plan*)plan;
     hipStream_t s = (hipStream_t)stream;
+    p.last_stream = s;
+    p.used = true;
     int rc;
     if (p.fwd_only) return fail(TW_EINVAL, "this plan was built forward_only");
     for (int k = 0; k < p.nshots; ++k)
@@ -1497,7 +1584,16 @@ int tw_plan_backward(void* plan, int accumulate, void* stream)
     return TW_OK;
 }
 
-void tw_plan_destroy(void* plan) { delete (Plan*)plan; }
+void tw_plan_destroy(void* plan)
+{
+    Plan* p = (Plan*)plan;
+    if (p && p->static_dev >= 0) {
+        // the next holder overwrites the workspace: let this plan's work finish first
+        if (p->used) (void)hipStreamSynchronize(p->last_stream);
+        static_release(p, p->static_dev);
+    }
+    delete p;
+}
 
 int tw_plan_set_timing(void* plan, int enable)
 {

@@ -128,11 +128,15 @@ class ShotPlan:
             gradient, the group's first shot's grad_of) is refused by the
             library for now: its adjoint kernel faulted on MI355X for a cause
             not yet found.
+    static_workspace: put the plan's descriptors in the library's static
+            per-device workspace (tw_plan_create_static: no device allocation
+            by the library; one such plan per device at a time, graph_steps
+            must be 0) instead of a workspace tensor.
     """
 
     def __init__(self, order, nt, h, dt, models, shots, ckpt_every, device, tile_rows=0,
                  graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK, forward_only=False, tile_cols=0,
-                 grad_group=1):
+                 grad_group=1, static_workspace=False):
         self.order, self.nt, self.h, self.dt = order, int(nt), float(h), float(dt)
         self.device = torch.device(device)
         self.K = int(ckpt_every)
@@ -224,14 +228,19 @@ class ShotPlan:
                 c.grad = self.grad.data_ptr() + 4 * o
         self._fs, self._nrec = fs, nrec
         L = _lib.lib()
-        nbytes = L.tw_plan_workspace_bytes(ctypes.byref(spec), cs, S)
-        if nbytes == 0:
-            raise RuntimeError("torchwi_amd: invalid shot plan: %s" % _lib.last_error())
-        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         handle = ctypes.c_void_p()
-        rc = L.tw_plan_create(ctypes.byref(spec), cs, S, _lib.ptr(self.workspace), nbytes,
-                              _lib.stream_of(self.workspace), ctypes.byref(handle))
-        _lib.check(rc, "tw_plan_create")
+        self.workspace = None
+        if static_workspace:
+            rc = L.tw_plan_create_static(ctypes.byref(spec), cs, S, _lib.stream_of(self.u), ctypes.byref(handle))
+            _lib.check(rc, "tw_plan_create_static")
+        else:
+            nbytes = L.tw_plan_workspace_bytes(ctypes.byref(spec), cs, S)
+            if nbytes == 0:
+                raise RuntimeError("torchwi_amd: invalid shot plan: %s" % _lib.last_error())
+            self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+            rc = L.tw_plan_create(ctypes.byref(spec), cs, S, _lib.ptr(self.workspace), nbytes,
+                                  _lib.stream_of(self.workspace), ctypes.byref(handle))
+            _lib.check(rc, "tw_plan_create")
         self.plan = handle
 
     # ---- views -------------------------------------------------------------
