@@ -66,6 +66,9 @@ int main(int argc, char** argv)
     spec.graph_steps = graph;
     spec.time_block = tb;
     spec.tile_rows = tile_rows;
+    // TW_EMU_GRAD_GROUP=G: the group adjoint (a library built with TW_TEST_GRAD_GROUP)
+    const char* gg = getenv("TW_EMU_GRAD_GROUP");
+    spec.grad_group = gg ? atoi(gg) : 0;
     std::vector<TwShotSpec> sh(S);
     for (int s = 0; s < S; ++s) {
         TwShotSpec& c = sh[s];

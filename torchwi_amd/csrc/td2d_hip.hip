@@ -773,6 +773,38 @@ int pipe_tiles(Plan& p, int chunk_rows)
     return TW_OK;
 }
 
+// The group adjoint's work list: every entry names a group leader whose
+// group lies inside the plan and shares the leader's grid, model and derived
+// arrays, every tile lies inside that grid, and every shot belongs to exactly
+// one group.
+int check_groups(const Plan& p, int tw, int ty)
+{
+    int covered = 0;
+    for (int s = 0; s < p.nshots; ++s) {
+        const ShotDesc& d = p.host[s];
+        if (d.gcount == 0) continue;
+        if (d.gcount < 0 || s + d.gcount > p.nshots)
+            return fail(TW_EINVAL, "internal: group of shot %d (%d shots) leaves the plan", s, d.gcount);
+        for (int j = 1; j < d.gcount; ++j) {
+            const ShotDesc& q = p.host[s + j];
+            if (q.gcount != 0 || q.vel != d.vel || q.vm != d.vm || q.vc3 != d.vc3 || q.dimx != d.dimx ||
+                q.dimy != d.dimy || q.ld != d.ld || q.fstride != d.fstride)
+                return fail(TW_EINVAL, "internal: shot %d does not match its group leader %d", s + j, s);
+        }
+        covered += d.gcount;
+    }
+    if (covered != p.nshots) return fail(TW_EINVAL, "internal: groups cover %d of %d shots", covered, p.nshots);
+    for (const int2& g : p.gtiles) {
+        if (g.x < 0 || g.x >= p.nshots || p.host[g.x].gcount < 1)
+            return fail(TW_EINVAL, "internal: group tile names shot %d, not a group leader", g.x);
+        const ShotDesc& d = p.host[g.x];
+        const int bx = g.y & 0xffff, by = g.y >> 16;
+        if (bx * tw >= d.dimx || by * ty >= d.dimy)
+            return fail(TW_EINVAL, "internal: group tile (%d, %d) outside the %dx%d grid", bx, by, d.dimx, d.dimy);
+    }
+    return TW_OK;
+}
+
 int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan& p)
 {
     if (!spec || !shots || nshots <= 0) return fail(TW_EINVAL, "empty shot list");
@@ -866,9 +898,12 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
     // Switched off: the first MI355X run of the group adjoint (k_vadj_group)
     // ended in a GPU memory fault whose cause has not been found (a CPU build
     // of this file's plan code and kernels under AddressSanitizer runs the
-    // same plan cleanly and reproduces the per-shot gradient).
+    // same plan cleanly and reproduces the per-shot gradient; only that CPU
+    // test build defines TW_TEST_GRAD_GROUP).
+#ifndef TW_TEST_GRAD_GROUP
     if (p.ggroup > 1)
         return fail(TW_EINVAL, "grad_group > 1 is switched off (its adjoint kernel faulted on MI355X; cause not found)");
+#endif
     if (p.tb > 1) return pipe_tiles(p, spec->tile_rows);
     if (spec->tile_cols != 0 && spec->tile_cols != 64 && spec->tile_cols != 128)
         return fail(TW_EINVAL, "tile_cols must be 0, 64 or 128 (got %d)", spec->tile_cols);
@@ -922,6 +957,7 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
         }
         p.gntiles = (int)p.gtiles.size();
         p.gper_xcd = (p.gntiles + NXCD - 1) / NXCD;
+        if ((rc = check_groups(p, tw, ty))) return rc;
     }
     return TW_OK;
 }
@@ -1257,6 +1293,9 @@ int place_and_upload(Plan& p, void* workspace, hipStream_t s)
     p.d_ctl = (int*)((char*)workspace + align256(sizeof(ShotDesc) * p.nshots) + align256(sizeof(int2) * p.ntiles));
     p.d_gtiles = (int2*)((char*)p.d_ctl + 256);
     p.d_vtiles = (int2*)((char*)p.d_gtiles + align256(sizeof(int2) * p.gntiles));
+    const char* end = (const char*)workspace + plan_bytes(p);
+    if ((const char*)(p.d_gtiles + p.gntiles) > end || (const char*)(p.d_vtiles + p.vtiles.size()) > end)
+        return fail(TW_EINVAL, "internal: tile lists overrun the %zu-byte workspace", plan_bytes(p));
     hipError_t e = hipMemcpyAsync(p.d_desc, p.host.data(), sizeof(ShotDesc) * p.nshots, hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
         e = hipMemcpyAsync(p.d_tiles, p.tiles.data(), sizeof(int2) * p.ntiles, hipMemcpyHostToDevice, s);
