@@ -50,6 +50,11 @@ inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int, int, boo
     pthread_barrier_wait(&g_wbar[w]);
     return v;
 }
+// mov_dpp with bound_ctrl: a lane with no source receives 0
+inline int __builtin_amdgcn_mov_dpp(int src, int ctrl, int rm, int bm, bool bound_ctrl)
+{
+    return __builtin_amdgcn_update_dpp(0, src, ctrl, rm, bm, bound_ctrl);
+}
 struct dim3 { unsigned x, y, z; dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {} };
 typedef int hipError_t;
 enum { hipSuccess = 0 };

@@ -667,8 +667,12 @@ void choose_tile(int& cx4, int& ryv)
     ryv = 1;
 }
 
-// Resident workgroups of the temporally blocked kernel on the device (the
-// least over its step variants), for the work-list sizing below.
+// Resident workgroups of the temporally blocked kernel's forward-step
+// variants on the device (the least of them), for the work-list sizing below.
+// The adjoint, capped at 3 workgroups per CU by its LDS, is left out: sized by
+// its 768 slots, the 64-shot 3008 x 1008 gradient took 168-row chunks and
+// 11.19 s; by the forward steps' 1024, 144-row chunks and 11.00 s (nt = 8000,
+// MI355X; the adjoint itself 1570 vs 1560 us per launch).
 template <int NE, int T>
 int pipe_slots_ne()
 {
@@ -677,7 +681,7 @@ int pipe_slots_ne()
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         return 0;
     const void* ks[] = {(const void*)k_pipe<NE, M_FWD, T>, (const void*)k_pipe<NE, M_FWD_VIRT, T>,
-                        (const void*)k_pipe<NE, M_RECOMP, T>, (const void*)k_pipe<NE, M_ADJ, T>};
+                        (const void*)k_pipe<NE, M_RECOMP, T>};
     for (const void* k : ks) {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 64 * (T + 1), 0) != hipSuccess || n <= 0)

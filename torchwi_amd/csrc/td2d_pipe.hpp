@@ -70,10 +70,12 @@ struct PLayout {
     static constexpr int CUR = 0;                          // ring k: NCUR rows of level k
     static constexpr int PRV = CUR + T * NCUR * PLANES;    // ring k: NPRV rows of level k-1
     static constexpr int GRD = PRV + T * NPRV * PLANES;    // ring k-1: NGRD rows of gradient sums after level k
-    // adjoint: ring k-1 holds the model rows of level k, passed to level k+1
-    // (read LAG iterations after the write, like the u_prev ring)
+    // ring k-1 holds the model rows of level k, passed to level k+1 (read LAG
+    // iterations after the write, like the u_prev ring); the recomputation and
+    // the adjoint only
+    static constexpr bool MRING = (FLAGS & (F_GRAD | F_VIRT)) != 0;
     static constexpr int MDL = GRD + ((FLAGS & F_GRAD) != 0 ? (T - 1) * NGRD * PLANES : 0);
-    static constexpr int KS = MDL + ((FLAGS & F_GRAD) != 0 ? (T - 1) * NPRV * PLANES : 0);  // per level: Keys scratch
+    static constexpr int KS = MDL + (MRING ? (T - 1) * NPRV * PLANES : 0);  // per level: Keys scratch
     static constexpr int SIZE0 = KS + T * 32;
     // The adjoint's LDS is at least 41.6 KB so that at most 3 workgroups (3
     // waves per SIMD) share a CU: at 4 (its VGPRs would allow them) it
@@ -88,14 +90,15 @@ struct IC {
 };
 
 // Wavefront shifts: lane l receives the value of lane l-1 (from_left) or of
-// lane l+1 (from_right); the lanes at the ends receive 0.
+// lane l+1 (from_right); the lanes at the ends receive 0 (bound_ctrl), so no
+// "old" value has to be materialised in the destination first.
 __device__ __forceinline__ float dpp_from_left(float v)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float dpp_from_right(float v)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float4 left4(float4 v)
 {
@@ -187,8 +190,10 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
         }
     }
     // Each wave role runs its own copy of the row loop (ROLE 0: the loader;
-    // 1: wave 1 of the adjoint, which starts from the stored gradient; 2: the
-    // other level waves) and issues exactly the loads it uses, the same ones
+    // 1: wave 1 of the recomputation and the adjoint, which loads the model
+    // rows (and in the adjoint starts from the stored gradient); 2: the other
+    // level waves, which there take the model rows from the level before) and
+    // issues exactly the loads it uses, the same ones
     // at every iteration (from cell 0 when a row is outside its range), so
     // the in-order count of outstanding vector-memory operations is the same
     // on every path and the wait for one queue slot never waits for the slots
@@ -204,9 +209,9 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
         constexpr int ROLE = decltype(RC)::value;
         const int row = O + slot_i - LAG * k;
         const bool rowA = row >= aLo && row < aHi, rowB = row >= bLo && row < bHi;
-        // adjoint level waves after the first take the model row from the
-        // previous level through LDS
-        if constexpr ((FLAGS & F_GRAD) == 0 || ROLE != 2) A = gld4(pA + ((aLane && rowA) ? row * ld + gx : 0));
+        // level waves after the first take the model row from the previous
+        // level through LDS
+        if constexpr (ROLE != 2 || !LY::MRING) A = gld4(pA + ((aLane && rowA) ? row * ld + gx : 0));
         if constexpr ((FLAGS & F_GRAD) != 0) {
             const int ib = (useB && bLane && rowB) ? row * ld + gx : 0;
             if constexpr (ROLE == 0) B = gld4(pB + ib);
@@ -300,11 +305,10 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
             // would become four conditional 4-byte reads
             float4 Gk = make_float4(0.f, 0.f, 0.f, 0.f);
             if constexpr ((FLAGS & F_GRAD) != 0 && ROLE != 1) Gk = S[grdR + cPr * PLANES];
-            // model of row r: wave 1 of the adjoint (and every level wave of
-            // the forward steps) from its own prefetch queue, the later
-            // adjoint level waves from wave k-1 (written with its u_prev row)
+            // model of row r: wave 1 from its own prefetch queue, the later
+            // level waves from wave k-1 (written with its u_prev row)
             float4 Mk;
-            if constexpr ((FLAGS & F_GRAD) != 0 && ROLE == 2) Mk = S[mdlR + cPr * PLANES];
+            if constexpr (ROLE == 2 && LY::MRING) Mk = S[mdlR + cPr * PLANES];
             else Mk = qA[U % PQ];
             const int r = O + i - LAG * k;
             if (active && r >= Ok && r < Ek) {
@@ -458,7 +462,7 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
                 if (writes_next) {
                     S[curW + cC * PLANES] = un;
                     S[prvW + cP * PLANES] = C;
-                    if constexpr ((FLAGS & F_GRAD) != 0) S[mdlW + cP * PLANES] = M;
+                    if constexpr (LY::MRING) S[mdlW + cP * PLANES] = M;
                 }
             }
         }
@@ -483,15 +487,9 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
             body(RC, base + 9, IC<9>{});
         }
     };
-    if constexpr ((FLAGS & F_GRAD) != 0) {
-        if (k == 0) march(IC<0>{});
-        else if (k == 1) march(IC<1>{});
-        else march(IC<2>{});
-    } else if (k == 0) {
-        march(IC<0>{});
-    } else {
-        march(IC<2>{});
-    }
+    if (k == 0) march(IC<0>{});
+    else if (k == 1 && LY::MRING) march(IC<1>{});
+    else march(IC<2>{});
 }
 
 }  // namespace tw
