@@ -64,7 +64,10 @@ def parse():
                     help="time steps per launch (1: one step; 2..4: blocked kernel; 0: automatic by size)")
     ap.add_argument("--tile-rows", type=int, default=0,
                     help="row-chunk height of the blocked kernel (0: automatic)")
-    ap.add_argument("--no-timing", action="store_true", help="skip per-launch HIP-event timing")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="blocked kernel: 1 one stream, 2 two shot groups on two streams, 0 automatic")
+    ap.add_argument("--no-timing", action="store_true",
+                    help="skip the extra gradient timed per launch (the kernels' roofline)")
     ap.add_argument("--compare", type=int, default=1,
                     help="also time the 384 x 1024 x 32-shot gradient (full storage, checkpointed, plain form)")
     ap.add_argument("--serving-requests", type=int, default=12,
@@ -179,7 +182,11 @@ def load_traffic(name):
 
 def run_scale(args, dev):
     """The headline: `steps` timed checkpointed misfit gradients of the
-    1000 x 3000 model after `warmup` untimed ones."""
+    1000 x 3000 model after `warmup` untimed ones; then, for the kernels'
+    roofline, one more gradient with every shot on one stream and an event
+    pair around each launch (the headline's two shot groups overlap their
+    launches on two streams, so their per-launch durations would not describe
+    one kernel's work)."""
     import torch
     from torchwi_amd import Time2dBatch
     from torchwi_amd import workloads as W
@@ -190,8 +197,9 @@ def run_scale(args, dev):
     dimx, dimy = nx + order, ny + order
     tb = args.time_block or auto_time_block(S * dimx * dimy)
     K = args.ckpt_every or default_ckpt_interval(nt, tb)
-    m = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K, time_block=tb,
-                    tile_rows=args.tile_rows)
+    mk = lambda streams: Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev,  # noqa: E731
+                                     ckpt_every=K, time_block=tb, tile_rows=args.tile_rows, streams=streams)
+    m = mk(args.streams)
     vel = torch.from_numpy(c["vel"]).to(dev)
     d_obs = m.modeling(vel * 1.01, c["sx"], c["sy"], c["ry"]).clone()
     torch.cuda.synchronize()
@@ -199,11 +207,7 @@ def run_scale(args, dev):
         m.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
         torch.cuda.synchronize()
         print("bench: warm-up gradient %d done" % (k + 1), file=sys.stderr, flush=True)
-    st = m.plan
-    timing = not args.no_timing
-    if timing:
-        st.set_timing(True)
-        st.timing(reset=True)
+    groups = m.plan.ngroups
     lat = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -216,12 +220,30 @@ def run_scale(args, dev):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     assert np.isfinite(float(loss)) and bool(torch.isfinite(g).all())
-    r = {"S": S, "K": st.K, "T": st.time_block, "dimx": dimx, "dimy": dimy, "elapsed": elapsed, "lat": lat,
+    r = {"S": S, "K": m.plan.K, "T": m.plan.time_block, "dimx": dimx, "dimy": dimy, "elapsed": elapsed,
+         "lat": lat, "groups": groups,
          "cells_step": 2.0 * S * dimx * dimy * nt, "h": h, "dt": dt, "nt": nt, "order": order, "nx": nx, "ny": ny}
-    if timing:
+    if not args.no_timing:
+        if groups != 1:
+            g_ref = g.clone()
+            del m
+            torch.cuda.empty_cache()
+            m = mk(1)
+            m.prepare(vel, c["sx"], c["sy"], c["ry"])
+        st = m.plan
+        st.set_timing(True)
+        st.timing(reset=True)
+        t1 = time.perf_counter()
+        _, g1 = m.misfit_gradient(vel, c["sx"], c["sy"], c["ry"], d_obs)
+        torch.cuda.synchronize()
+        r["one_stream_s"] = time.perf_counter() - t1
         r["timing"] = st.timing()
         st.set_timing(False)
-    del m, st, d_obs
+        if groups != 1:
+            assert torch.equal(g1, g_ref), "one stream and two shot groups must give the same gradient"
+        print("bench: one-stream gradient timed per launch: %.3f s" % r["one_stream_s"], file=sys.stderr, flush=True)
+        del st
+    del m, d_obs
     torch.cuda.empty_cache()
     return r
 
@@ -357,7 +379,8 @@ def main():
                                % (S, r["ny"], r["nx"], r["dimy"], r["dimx"], r["h"], r["dt"], r["nt"], r["order"],
                                   r["K"], T),
                    "shots": S, "nz": r["ny"], "nx": r["nx"], "nt": r["nt"], "order": r["order"],
-                   "ckpt_every": r["K"], "time_block": T, "tile_rows": args.tile_rows},
+                   "ckpt_every": r["K"], "time_block": T, "tile_rows": args.tile_rows,
+                   "shot_groups": r["groups"]},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 1),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 1)},
         "gradient_shots_per_s": round(S * args.steps / elapsed, 3),
@@ -373,7 +396,8 @@ def main():
         out["roofline"] = {"bound": "hbm", "kernel": kname, "achieved": per[dom]["achieved_GBs"],
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": per[dom]["frac"], "traffic": None,
                            "bytes_per_cell_per_launch": per[dom]["bytes_per_cell_per_launch"],
-                           "cells_per_launch": cells, "steps_per_launch": T}
+                           "cells_per_launch": cells, "steps_per_launch": T,
+                           "measured_on": "one extra gradient, all shots on one stream, %.3f s" % r["one_stream_s"]}
         tr = load_traffic("traffic_k_pipe_1000x3000_64shots.json")
         if (tr and (S, r["nt"] >= 300, T, args.tile_rows) == (64, True, 3, 0)
                 and dom in tr.get("kernels", {})):

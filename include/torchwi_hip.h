@@ -127,6 +127,14 @@ typedef struct TwPlanSpec {
                          of one model into one gradient) is currently refused
                          with TW_EINVAL: its kernel faulted on MI355X for a
                          cause not yet found */
+    int streams;      /* time_block > 1: 1 runs all shots on the caller's
+                         stream; 2 runs them as two shot groups of about equal
+                         work, the second on a stream of the plan's own, its
+                         checkpoint segments half a segment behind the first
+                         (the caller's stream waits for both at the end of
+                         each pass); 0 = automatic: 2 whenever the plan
+                         has two shots or more.
+                         Results are the same either way */
 } TwPlanSpec;
 
 typedef struct TwShotSpec {
@@ -171,6 +179,10 @@ int tw_plan_backward(void *plan, int accumulate, void *stream);
 
 void tw_plan_destroy(void *plan);
 
+/* Shot groups the plan runs on separate streams (1 or 2, see
+ * TwPlanSpec.streams), or a negative status. */
+int tw_plan_shot_groups(void *plan);
+
 /* Size of the static plan workspace of each device (4 MiB). */
 #define TW_STATIC_WORKSPACE_BYTES 4194304
 
@@ -183,7 +195,9 @@ void tw_plan_destroy(void *plan);
  * (code that runs between graph captures, serving loops with a fixed memory
  * footprint, harnesses that forbid allocation).  Limits: the plan must fit
  * the workspace (plan sizes beyond it return TW_EINVAL; tw_plan_workspace_bytes
- * gives the size); graph_steps must be 0 (graph instantiation may allocate);
+ * gives the size); graph_steps must be 0 (graph instantiation may allocate)
+ * and streams 0 or 1 (a plan stream and events are not created: the plan runs
+ * on the caller's stream);
  * at most one such plan per device exists at a time -- a second
  * tw_plan_create_static on a device whose workspace is held returns
  * TW_EINVAL until the holder is destroyed.  The workspace keeps the last

@@ -69,6 +69,9 @@ int main(int argc, char** argv)
     // TW_EMU_GRAD_GROUP=G: the group adjoint (a library built with TW_TEST_GRAD_GROUP)
     const char* gg = getenv("TW_EMU_GRAD_GROUP");
     spec.grad_group = gg ? atoi(gg) : 0;
+    // TW_EMU_STREAMS=2: the blocked kernel's two shot groups
+    const char* sm = getenv("TW_EMU_STREAMS");
+    spec.streams = sm ? atoi(sm) : 0;
     std::vector<TwShotSpec> sh(S);
     for (int s = 0; s < S; ++s) {
         TwShotSpec& c = sh[s];

@@ -114,6 +114,10 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms =
 inline hipError_t hipEventDestroy(hipEvent_t) { return 0; }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, int) { *s = (void*)1; return 0; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return 0; }
+// launches run synchronously in enqueue order, so stream order and event waits hold trivially
+enum { hipEventDisableTiming = 2 };
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { *e = nullptr; return 0; }
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return 0; }
 inline hipError_t hipStreamBeginCapture(hipStream_t, int) { g_capture = new std::vector<std::function<void()>>(); return 0; }
 inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) { *g = g_capture; g_capture = nullptr; return 0; }
 inline hipError_t hipGraphInstantiate(hipGraphExec_t* x, hipGraph_t g, void*, void*, size_t) { *x = g; return 0; }

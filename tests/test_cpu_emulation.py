@@ -48,7 +48,9 @@ def driver(tmp_path_factory):
     (8, 24, 14, 9, 2, 0, 0, 3, 0, 0), (8, 250, 26, 6, 1, 0, 0, 3, 16, 0), (4, 40, 22, 6, 2, 4, 0, 2, 0, 0),
     (2, 33, 21, 8, 2, 4, 0, 4, 0, 0), (4, 40, 70, 6, 2, 3, 0, 3, 0, 0),
     # the plan descriptors in the static workspace (tw_plan_create_static)
-    (8, 24, 14, 9, 2, 3, 0, 3, 0, 1), (4, 30, 16, 8, 3, 0, 0, 1, 0, 1)])
+    (8, 24, 14, 9, 2, 3, 0, 3, 0, 1), (4, 30, 16, 8, 3, 0, 0, 1, 0, 1),
+    # the blocked kernel's two shot groups (static = 2: TW_EMU_STREAMS=2)
+    (8, 24, 14, 10, 3, 3, 0, 3, 0, 2), (4, 40, 22, 9, 5, 0, 0, 3, 0, 2), (2, 33, 21, 13, 2, 6, 0, 2, 0, 2)])
 def test_kernel_source_on_host_threads_matches_oracle(driver, tmp_path, order, nx, ny, nt, S, K, graph, tb, rows,
                                                       static):
     rng = np.random.default_rng(order * 7 + nx)
@@ -69,7 +71,8 @@ def test_kernel_source_on_host_threads_matches_oracle(driver, tmp_path, order, n
         np.array([h, dt], np.float32).tofile(f)
         for a in (vld, w, pos, resid):
             a.tofile(f)
-    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", TW_EMU_STATIC=str(static))
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", TW_EMU_STATIC=str(int(static == 1)),
+               TW_EMU_STREAMS=str(2 if static == 2 else 0))
     r = subprocess.run([driver, fin, fout], capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     out = np.fromfile(fout, np.float32)

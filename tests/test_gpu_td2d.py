@@ -482,6 +482,50 @@ def test_auto_time_block_large_plan_bitexact():
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("order,S,K,nt,windows", [(8, 5, 12, 40, False), (4, 4, 0, 30, False), (8, 6, 9, 28, True)])
+def test_two_shot_groups_on_two_streams_bitexact(order, S, K, nt, windows):
+    """The blocked kernel's two shot groups on two streams (streams = 2; the
+    second group's checkpoint segments half a segment behind the first) give
+    the gathers and per-shot gradients of one stream bit for bit, and those
+    equal the oracle's (shot 1)."""
+    from torchwi_amd import Time2dBatch, Time2dWindows
+    c = small_case(order, nx=150, ny=60, nt=nt, seed=S * 7 + K)
+    h = c["h"]
+    vel = torch.from_numpy(c["vel"]).to(DEV)
+    w = torch.from_numpy(c["w"])
+    sx = [(9 + 27 * s + 0.5) * h for s in range(S)]
+    sy = [(s % 3 + 0.5) * h for s in range(S)]
+    ry = [(1 + s % 2 + 0.5) * h for s in range(S)]
+    print(f"case two shot groups order={order} S={S} K={K} nt={nt} windows={windows}", flush=True)
+    res = []
+    for streams in (2, 1):
+        if windows:
+            wins = [dict(x0=10 * s, x1=10 * s + 90, sx=(20 + s + 0.5) * h, sy=sy[s], ry=ry[s], rx0=3, nrec=80)
+                    for s in range(S)]
+            m = Time2dWindows(c["nx"], c["ny"], h, w, c["dt"], order, wins, DEV, ckpt_every=K, time_block=3,
+                              streams=streams)
+            frd = [f.clone() for f in m.modeling(vel)]
+            m.misfit_gradient(vel, [0.5 * f for f in frd])
+            frd = torch.stack(frd)
+        else:
+            m = Time2dBatch(c["nx"], c["ny"], h, w, c["dt"], order, DEV, ckpt_every=K, time_block=3, streams=streams)
+            frd = m.modeling(vel, sx, sy, ry).clone()
+            m.misfit_gradient(vel, sx, sy, ry, 0.5 * frd)
+        assert m.plan.ngroups == streams
+        res.append((frd, [m.plan.grad_of(s).clone() for s in range(S)]))
+        del m
+    assert torch.equal(res[0][0], res[1][0]), "gathers"
+    for s in range(S):
+        assert torch.equal(res[0][1][s], res[1][1][s]), f"gradient of shot {s}"
+    if not windows:
+        vpad = O.pad_vel(c["vel"], order)
+        a = (order, c["dimx"], c["dimy"], h, c["dt"])
+        frd_o, virt_o = O.forward(vpad, c["w"], *a, sx[1], sy[1], ry[1])
+        assert np.array_equal(res[0][0][1].cpu().numpy(), frd_o)
+        g_o = O.backward(virt_o, vpad, 0.5 * frd_o, *a, ry[1]).reshape(c["dimy"], c["dimx"])
+        assert np.array_equal(res[0][1][1][:, :c["dimx"]].cpu().numpy(), g_o)
+
+
 def test_stream_copy():
     """The measurement copy (tw_stream_copy) copies exactly, including a
     size that is not a multiple of the workgroup, and refuses sizes and

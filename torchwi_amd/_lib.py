@@ -34,6 +34,7 @@ SIGNATURES = {
     "tw_plan_forward": (_ci, [_vp, _vp]),
     "tw_plan_backward": (_ci, [_vp, _ci, _vp]),
     "tw_plan_destroy": (None, [_vp]),
+    "tw_plan_shot_groups": (_ci, [_vp]),
     "tw_plan_create_static": (_ci, [_vp, _vp, _ci, _vp, ctypes.POINTER(_vp)]),
     "tw_plan_set_timing": (_ci, [_vp, _ci]),
     "tw_plan_timing": (_ci, [_vp, _ci, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]),
@@ -47,7 +48,7 @@ KINDS = ("fwd", "fwd_virt", "recomp", "adj")
 class TwPlanSpec(ctypes.Structure):
     _fields_ = [("order", _ci), ("nt", _ci), ("h", _cf), ("dt", _cf), ("ckpt_every", _ci),
                 ("tile_rows", _ci), ("graph_steps", _ci), ("time_block", _ci), ("forward_only", _ci),
-                ("tile_cols", _ci), ("grad_group", _ci)]
+                ("tile_cols", _ci), ("grad_group", _ci), ("streams", _ci)]
 
 
 class TwShotSpec(ctypes.Structure):

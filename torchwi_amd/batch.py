@@ -95,6 +95,7 @@ class Time2dBatch(torch.nn.Module):
     ``time_block``: time steps per kernel launch; "auto" picks the
     temporally blocked kernel (3 steps) for plans of at least
     shots.AUTO_BLOCK_CELLS cells per step and one step per launch below.
+    ``streams``: the blocked kernel's shot groups (shots.ShotPlan; 0 automatic).
     ``grad_group``: shots per gradient group of the one-step adjoint
     (shots.ShotPlan; values > 1 are refused for now); 1 keeps one gradient
     per shot, whose sum in shot order is the returned gradient.
@@ -104,8 +105,9 @@ class Time2dBatch(torch.nn.Module):
 
     def __init__(self, nx, ny, h, w, dt, order, device="cuda", ckpt_every="auto",
                  hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS, time_block="auto",
-                 tile_cols=0, grad_group=1):
+                 tile_cols=0, grad_group=1, streams=0):
         super().__init__()
+        self.streams = int(streams)
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise NotImplementedError("Time2dBatch runs on the GPU only")
@@ -146,7 +148,7 @@ class Time2dBatch(torch.nn.Module):
             self._plan = ShotPlan(self.order, self.nt, self.h, self.dt, [(self.nx, self.ny)], shots, K,
                                   self.device, tile_rows=self.tile_rows, graph_steps=self.graph_steps,
                                   time_block=tb, tile_cols=self.tile_cols,
-                                  grad_group=self.grad_group if tb == 1 else 1)
+                                  grad_group=self.grad_group if tb == 1 else 1, streams=self.streams)
             self._key = key
         self._plan.models[0].load(vel)
         return self._plan
@@ -204,7 +206,7 @@ class Time2dWindows(torch.nn.Module):
 
     def __init__(self, nx, ny, h, w, dt, order, windows, device="cuda", ckpt_every="auto",
                  hbm_fraction=0.8, tile_rows=0, graph_steps=GRAPH_STEPS, time_block="auto",
-                 tile_cols=0):
+                 tile_cols=0, streams=0):
         super().__init__()
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -224,7 +226,8 @@ class Time2dWindows(torch.nn.Module):
         K = _choose_ckpt(ckpt_every, plan_bytes([(a, b, order) for a, b in models], shots, self.nt, 0),
                          self.nt, self.device, hbm_fraction, time_block=time_block)
         self.plan = ShotPlan(order, self.nt, h, dt, models, shots, K, self.device, tile_rows=tile_rows,
-                             graph_steps=graph_steps, time_block=time_block, tile_cols=tile_cols)
+                             graph_steps=graph_steps, time_block=time_block, tile_cols=tile_cols,
+                             streams=streams)
 
     def load(self, vel):
         for i, q in enumerate(self.windows):

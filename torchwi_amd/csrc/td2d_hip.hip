@@ -20,6 +20,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -280,13 +281,13 @@ __global__ void k_zero_i16(short* __restrict__ p, long long n)
 // step t0: u_prev <- ckpt[seg][0], u_cur <- ckpt[seg][1] (zero for seg 0);
 // or (what = 1) zero the whole adjoint ring; or (what = 2) zero grad.
 __global__ void k_batch_prepare(const ShotDesc* __restrict__ descs, int what, int seg, int t0,
-                                const int* __restrict__ ctl, int K, int pairs)
+                                const int* __restrict__ ctl, int K, int pairs, int shot0)
 {
     if (ctl) {  // segment index from the control word (graph replay)
         seg = *ctl;
         t0 = seg * K;
     }
-    const ShotDesc& d = descs[blockIdx.y];
+    const ShotDesc& d = descs[shot0 + blockIdx.y];
     const long long n = (long long)d.dimy * d.ld;
     const long long stride = (long long)gridDim.x * blockDim.x;
     const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -640,11 +641,23 @@ struct Plan {
     int2* d_tiles = nullptr;
     int2* d_gtiles = nullptr;
     int2* d_vtiles = nullptr;
+    // Blocked plans may run their shots as two groups on two streams (the
+    // caller's and s1), whose launches overlap on the device; group g holds
+    // shots [gs[g], gs[g+1]) and the tile-list ranges [t_off[g], t_off[g+1])
+    // and [vt_off[g], vt_off[g+1]).
+    int ngroups = 1;
+    int gsplit = 0;
+    int t_off[3] = {0, 0, 0}, vt_off[3] = {0, 0, 0};
+    hipStream_t s1 = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int static_dev = -1;              // >= 0: the descriptors live in that device's static workspace
     hipStream_t last_stream = nullptr;  // stream of the latest forward / backward pass
     bool used = false;
     ~Plan()
     {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (s1) (void)hipStreamDestroy(s1);
         g_fwd.release();
         g_bwd.release();
         g_recomp.release();
@@ -809,6 +822,47 @@ int check_groups(const Plan& p, int tw, int ty)
     return TW_OK;
 }
 
+// Split a blocked plan's shots into two groups of about equal work (tile
+// count), each group's entries contiguous in both work lists, or keep one
+// group.  streams: 1 one group, 2 two, 0 automatic (two whenever there are
+// two shots).  Two groups on two streams overlap one group's memory-bound
+// adjoint with the other's recomputation and fill each launch's tail with the
+// other group's work.  Measured on MI355X, checkpointed gradients, one group /
+// two: 64 shots of 3008 x 1008, nt = 8000: 10.97 / 10.20 s; 16 shots,
+// nt = 3000: 1.266 / 1.165 s; 8 shots: 0.727 / 0.625 s; 256 uneven windows of
+// the 600 x 4000 model, nt = 6000: 10.68 / 9.84 s.
+int split_groups(Plan& p, int streams)
+{
+    const int n = p.ntiles;
+    p.ngroups = 1;
+    p.gsplit = p.nshots;
+    p.t_off[0] = p.vt_off[0] = 0;
+    p.t_off[1] = p.t_off[2] = n;
+    p.vt_off[1] = p.vt_off[2] = (int)p.vtiles.size();
+    if (streams < 0 || streams > 2) return fail(TW_EINVAL, "streams must be 0, 1 or 2 (got %d)", streams);
+    if (streams == 1 || p.nshots < 2) return TW_OK;
+    // the shot at which the cumulative tile count first reaches half
+    std::vector<int> per(p.nshots, 0);
+    for (const int2& t : p.tiles) per[t.x]++;
+    int cut = 1, acc = 0;
+    for (int k = 0; k < p.nshots - 1; ++k) {
+        acc += per[k];
+        cut = k + 1;
+        if (2 * acc >= n) break;
+    }
+    int tiles0 = 0;
+    for (int k = 0; k < cut; ++k) tiles0 += per[k];
+    auto part = [&](std::vector<int2>& v) {
+        std::stable_partition(v.begin(), v.end(), [&](const int2& t) { return t.x < cut; });
+    };
+    part(p.tiles);
+    part(p.vtiles);
+    p.ngroups = 2;
+    p.gsplit = cut;
+    p.t_off[1] = p.vt_off[1] = tiles0;
+    return TW_OK;
+}
+
 int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan& p)
 {
     if (!spec || !shots || nshots <= 0) return fail(TW_EINVAL, "empty shot list");
@@ -908,7 +962,10 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
     if (p.ggroup > 1)
         return fail(TW_EINVAL, "grad_group > 1 is switched off (its adjoint kernel faulted on MI355X; cause not found)");
 #endif
-    if (p.tb > 1) return pipe_tiles(p, spec->tile_rows);
+    if (p.tb > 1) {
+        if ((rc = pipe_tiles(p, spec->tile_rows))) return rc;
+        return split_groups(p, spec->streams);
+    }
     if (spec->tile_cols != 0 && spec->tile_cols != 64 && spec->tile_cols != 128)
         return fail(TW_EINVAL, "tile_cols must be 0, 64 or 128 (got %d)", spec->tile_cols);
     if (spec->tile_rows != 0 && spec->tile_rows != 8 && spec->tile_rows != 16 && spec->tile_rows != 32 &&
@@ -1145,29 +1202,32 @@ int replay(Plan& p, Graph& gr, int kind, int steps, hipStream_t s)
 // 3608 x 1208); forward 406 vs 390 and recomputation 558 vs 569 (64 of
 // 3008 x 1008), recomputation 177 vs 165 (12 of 3608 x 1208).
 template <int NE, unsigned FLAGS, int T>
-void launch_t_nft(const Plan& p, const TArgs& a, hipStream_t s)
+void launch_t_nft(const Plan& p, const TArgs& a, hipStream_t s, int group)
 {
+    // the adjoint's list is shot-major, the forward steps' chunk-major; a
+    // shot group's entries form one contiguous range of either list
     const int2* tl = (FLAGS & F_GRAD) != 0 ? p.d_vtiles : p.d_tiles;
-    hipLaunchKernelGGL((k_pipe<NE, FLAGS, T>), dim3(p.per_xcd * NXCD), dim3(64 * (T + 1)), 0, s, p.d_desc,
+    if (group >= 0) tl += (FLAGS & F_GRAD) != 0 ? p.vt_off[group] : p.t_off[group];
+    hipLaunchKernelGGL((k_pipe<NE, FLAGS, T>), dim3(a.per_xcd * NXCD), dim3(64 * (T + 1)), 0, s, p.d_desc,
                        tl, a);
 }
 
 template <int NE, unsigned FLAGS>
-void launch_t_nf(const Plan& p, const TArgs& a, hipStream_t s)
+void launch_t_nf(const Plan& p, const TArgs& a, hipStream_t s, int group)
 {
-    if (p.tb == 2) launch_t_nft<NE, FLAGS, 2>(p, a, s);
-    else if (p.tb == 3) launch_t_nft<NE, FLAGS, 3>(p, a, s);
-    else launch_t_nft<NE, FLAGS, 4>(p, a, s);
+    if (p.tb == 2) launch_t_nft<NE, FLAGS, 2>(p, a, s, group);
+    else if (p.tb == 3) launch_t_nft<NE, FLAGS, 3>(p, a, s, group);
+    else launch_t_nft<NE, FLAGS, 4>(p, a, s, group);
 }
 
 template <int NE>
-int launch_t_ne(const Plan& p, const TArgs& a, unsigned flags, hipStream_t s)
+int launch_t_ne(const Plan& p, const TArgs& a, unsigned flags, hipStream_t s, int group)
 {
     switch (flags) {
-    case M_FWD: launch_t_nf<NE, M_FWD>(p, a, s); break;
-    case M_FWD_VIRT: launch_t_nf<NE, M_FWD_VIRT>(p, a, s); break;
-    case M_RECOMP: launch_t_nf<NE, M_RECOMP>(p, a, s); break;
-    case M_ADJ: launch_t_nf<NE, M_ADJ>(p, a, s); break;
+    case M_FWD: launch_t_nf<NE, M_FWD>(p, a, s, group); break;
+    case M_FWD_VIRT: launch_t_nf<NE, M_FWD_VIRT>(p, a, s, group); break;
+    case M_RECOMP: launch_t_nf<NE, M_RECOMP>(p, a, s, group); break;
+    case M_ADJ: launch_t_nf<NE, M_ADJ>(p, a, s, group); break;
     default: return fail(TW_EINVAL, "internal: no blocked step variant for flags %u", flags);
     }
     return TW_OK;
@@ -1176,7 +1236,7 @@ int launch_t_ne(const Plan& p, const TArgs& a, unsigned flags, hipStream_t s)
 // Launch one temporally blocked step of `n` levels starting at time t
 // (advancing by dir), reading field pair in_pair and writing the other.
 int launch_block(Plan& p, unsigned flags, int t, int dir, int n, int in_pair, int vslot, int adj,
-                 hipStream_t s)
+                 hipStream_t s, int group = -1)
 {
     TArgs a;
     std::memset(&a, 0, sizeof a);
@@ -1190,13 +1250,17 @@ int launch_block(Plan& p, unsigned flags, int t, int dir, int n, int in_pair, in
     a.K = p.K;
     a.nseg = p.nseg;
     a.ntiles = p.ntiles;
-    a.per_xcd = p.per_xcd;
+    if (group >= 0) {
+        const bool v = (flags & F_GRAD) != 0;
+        a.ntiles = (v ? p.vt_off : p.t_off)[group + 1] - (v ? p.vt_off : p.t_off)[group];
+    }
+    a.per_xcd = (a.ntiles + NXCD - 1) / NXCD;
     EventPair* ev = p.timer.enabled ? p.timer.next(s) : nullptr;
     if (ev) TW_HIP(hipEventRecord(ev->start, s));
     int rc;
-    if (p.order == 2) rc = launch_t_ne<1>(p, a, flags, s);
-    else if (p.order == 4) rc = launch_t_ne<2>(p, a, flags, s);
-    else rc = launch_t_ne<4>(p, a, flags, s);
+    if (p.order == 2) rc = launch_t_ne<1>(p, a, flags, s, group);
+    else if (p.order == 4) rc = launch_t_ne<2>(p, a, flags, s, group);
+    else rc = launch_t_ne<4>(p, a, flags, s, group);
     if (rc) return rc;
     TW_HIP(hipGetLastError());
     if (ev) {
@@ -1221,13 +1285,33 @@ int prep_models(const Plan& p, hipStream_t s)
     return TW_OK;
 }
 
-int batch_prepare(Plan& p, int what, int seg, int t0, hipStream_t s)
+// group: -1 every shot, else that shot group's shots only
+int batch_prepare(Plan& p, int what, int seg, int t0, hipStream_t s, int group = -1)
 {
     long long bx = (p.max_field + 255) / 256;
     if (bx > 1024) bx = 1024;
-    hipLaunchKernelGGL(k_batch_prepare, dim3((unsigned)bx, p.nshots), dim3(256), 0, s,
-                       p.d_desc, what, seg, t0, (const int*)nullptr, p.K, p.tb > 1 ? 1 : 0);
+    const int s0 = group == 1 ? p.gsplit : 0;
+    const int ns = group < 0 ? p.nshots : (group == 0 ? p.gsplit : p.nshots - p.gsplit);
+    hipLaunchKernelGGL(k_batch_prepare, dim3((unsigned)bx, ns), dim3(256), 0, s,
+                       p.d_desc, what, seg, t0, (const int*)nullptr, p.K, p.tb > 1 ? 1 : 0, s0);
     TW_HIP(hipGetLastError());
+    return TW_OK;
+}
+
+// The second stream and the events of a two-group plan (created once).
+int group_streams(Plan& p)
+{
+    if (p.s1) return TW_OK;
+    TW_HIP(hipStreamCreateWithFlags(&p.s1, hipStreamNonBlocking));
+    for (hipEvent_t& e : p.ev) TW_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return TW_OK;
+}
+
+// s_to waits for the work enqueued so far on s_from (through event e).
+int join(Plan& p, int e, hipStream_t s_from, hipStream_t s_to)
+{
+    TW_HIP(hipEventRecord(p.ev[e], s_from));
+    TW_HIP(hipStreamWaitEvent(s_to, p.ev[e], 0));
     return TW_OK;
 }
 
@@ -1237,7 +1321,7 @@ int batch_prepare_ctl(Plan& p, hipStream_t s)
     long long bx = (p.max_field + 255) / 256;
     if (bx > 1024) bx = 1024;
     hipLaunchKernelGGL(k_batch_prepare, dim3((unsigned)bx, p.nshots), dim3(256), 0, s,
-                       p.d_desc, 0, 0, 0, (const int*)(p.d_ctl + 1), p.K, 0);
+                       p.d_desc, 0, 0, 0, (const int*)(p.d_ctl + 1), p.K, 0, 0);
     TW_HIP(hipGetLastError());
     return TW_OK;
 }
@@ -1459,8 +1543,15 @@ int tw_plan_create_static(const TwPlanSpec* spec, const TwShotSpec* shots, int n
     if (spec && spec->graph_steps != 0)
         return fail(TW_EINVAL, "tw_plan_create_static takes graph_steps = 0 (graph capture and "
                                "instantiation may allocate device memory)");
+    if (spec && spec->streams == 2)
+        return fail(TW_EINVAL, "tw_plan_create_static takes streams = 0 or 1 (it creates no stream)");
+    TwPlanSpec one;
+    if (spec) {
+        one = *spec;
+        one.streams = 1;
+    }
     Plan* p = new Plan();
-    int rc = build_plan(spec, shots, nshots, *p);
+    int rc = build_plan(spec ? &one : nullptr, shots, nshots, *p);
     if (rc) { delete p; return rc; }
     const size_t need = plan_bytes(*p);
     if (need > STATIC_WS_BYTES) {
@@ -1498,6 +1589,19 @@ int tw_plan_forward(void* plan, void* stream)
     const int kind = store ? TW_KIND_FWD_VIRT : TW_KIND_FWD;
     if (p.tb > 1) {
         const unsigned flags = store ? M_FWD_VIRT : M_FWD;
+        if (p.ngroups == 2) {
+            // the two groups march independently, one per stream
+            if ((rc = group_streams(p)) || (rc = join(p, 0, s, p.s1))) return rc;
+            for (int g = 0; g < 2; ++g) {
+                hipStream_t sg = g == 0 ? s : p.s1;
+                int b = 0;
+                for (int t0 = 0; t0 < p.nt; t0 += p.tb, ++b) {
+                    const int n = p.nt - t0 < p.tb ? p.nt - t0 : p.tb;
+                    if ((rc = launch_block(p, flags, t0, +1, n, b & 1, t0, 0, sg, g))) return rc;
+                }
+            }
+            return join(p, 1, p.s1, s);
+        }
         int b = 0;
         for (int t0 = 0; t0 < p.nt; t0 += p.tb, ++b) {
             const int n = p.nt - t0 < p.tb ? p.nt - t0 : p.tb;
@@ -1548,30 +1652,51 @@ int tw_plan_backward(void* plan, int accumulate, void* stream)
     if (!accumulate && (rc = batch_prepare(p, 2, 0, 0, s))) return rc;
     if ((rc = batch_prepare(p, 1, 0, 0, s))) return rc;
     if (p.tb > 1) {
-        int ab = 0;  // adjoint blocks done: parity of the adjoint field pair
+        const bool two = p.ngroups == 2;
+        if (two && ((rc = group_streams(p)) || (rc = join(p, 0, s, p.s1)))) return rc;
+        const int ng = two ? 2 : 1;
         if (p.K == 0) {
-            for (int s0 = 0; s0 < p.nt; s0 += p.tb, ++ab) {
-                const int n = p.nt - s0 < p.tb ? p.nt - s0 : p.tb;
-                const int it = p.nt - 1 - s0;
-                if ((rc = launch_block(p, M_ADJ, it, -1, n, ab & 1, it, 1, s))) return rc;
+            for (int g = 0; g < ng; ++g) {
+                hipStream_t sg = g == 0 ? s : p.s1;
+                int ab = 0;  // adjoint blocks done: parity of the adjoint field pair
+                for (int s0 = 0; s0 < p.nt; s0 += p.tb, ++ab) {
+                    const int n = p.nt - s0 < p.tb ? p.nt - s0 : p.tb;
+                    const int it = p.nt - 1 - s0;
+                    if ((rc = launch_block(p, M_ADJ, it, -1, n, ab & 1, it, 1, sg, two ? g : -1))) return rc;
+                }
             }
-            return TW_OK;
+            return two ? join(p, 1, p.s1, s) : TW_OK;
         }
+        // Checkpoint segments last to first, each a recomputation then the
+        // adjoint.  With two groups the second lags the first by half a
+        // segment: group 1 recomputes segment seg while group 0 runs its
+        // adjoint, and group 0 recomputes seg-1 while group 1 runs its
+        // adjoint (events 2 and 3 order the recomputations).
+        int ab = 0;
         for (int seg = p.nseg - 1; seg >= 0; --seg) {
             const int t0 = seg * p.K;
             const int t1 = (t0 + p.K < p.nt) ? t0 + p.K : p.nt;
-            if ((rc = batch_prepare(p, 0, seg, t0, s))) return rc;
-            int b = 0;
-            for (int t = t0; t < t1; t += p.tb, ++b) {
-                const int n = t1 - t < p.tb ? t1 - t : p.tb;
-                if ((rc = launch_block(p, M_RECOMP, t, +1, n, b & 1, t - t0, 0, s))) return rc;
-            }
-            for (int it = t1 - 1; it >= t0; it -= p.tb, ++ab) {
-                const int n = it - t0 + 1 < p.tb ? it - t0 + 1 : p.tb;
-                if ((rc = launch_block(p, M_ADJ, it, -1, n, ab & 1, it - t0, 1, s))) return rc;
+            for (int g = 0; g < ng; ++g) {
+                hipStream_t sg = g == 0 ? s : p.s1;
+                const int grp = two ? g : -1;
+                if (two && g == 0 && seg < p.nseg - 1) TW_HIP(hipStreamWaitEvent(s, p.ev[3], 0));
+                if (two && g == 1) TW_HIP(hipStreamWaitEvent(p.s1, p.ev[2], 0));
+                if ((rc = batch_prepare(p, 0, seg, t0, sg, grp))) return rc;
+                int b = 0;
+                for (int t = t0; t < t1; t += p.tb, ++b) {
+                    const int n = t1 - t < p.tb ? t1 - t : p.tb;
+                    if ((rc = launch_block(p, M_RECOMP, t, +1, n, b & 1, t - t0, 0, sg, grp))) return rc;
+                }
+                if (two) TW_HIP(hipEventRecord(p.ev[g == 0 ? 2 : 3], sg));
+                int a2 = ab;
+                for (int it = t1 - 1; it >= t0; it -= p.tb, ++a2) {
+                    const int n = it - t0 + 1 < p.tb ? it - t0 + 1 : p.tb;
+                    if ((rc = launch_block(p, M_ADJ, it, -1, n, a2 & 1, it - t0, 1, sg, grp))) return rc;
+                }
+                if (g == ng - 1) ab = a2;
             }
         }
-        return TW_OK;
+        return two ? join(p, 1, p.s1, s) : TW_OK;
     }
     if (p.graph_steps > 0) {
         if (p.K == 0) {
@@ -1636,6 +1761,12 @@ void tw_plan_destroy(void* plan)
         static_release(p, p->static_dev);
     }
     delete p;
+}
+
+int tw_plan_shot_groups(void* plan)
+{
+    if (!plan) return fail(TW_EINVAL, "NULL plan");
+    return ((Plan*)plan)->ngroups;
 }
 
 int tw_plan_set_timing(void* plan, int enable)

@@ -128,6 +128,10 @@ class ShotPlan:
             gradient, the group's first shot's grad_of) is refused by the
             library for now: its adjoint kernel faulted on MI355X for a cause
             not yet found.
+    streams: blocked plans only: 1 all shots on the caller's stream, 2 two
+            shot groups on two streams whose launches overlap (the second group
+            half a checkpoint segment behind), 0 automatic (two whenever there
+            are two shots).  The results are the same.
     static_workspace: put the plan's descriptors in the library's static
             per-device workspace (tw_plan_create_static: no device allocation
             by the library; one such plan per device at a time, graph_steps
@@ -136,7 +140,7 @@ class ShotPlan:
 
     def __init__(self, order, nt, h, dt, models, shots, ckpt_every, device, tile_rows=0,
                  graph_steps=GRAPH_STEPS, time_block=TIME_BLOCK, forward_only=False, tile_cols=0,
-                 grad_group=1, static_workspace=False):
+                 grad_group=1, static_workspace=False, streams=0):
         self.order, self.nt, self.h, self.dt = order, int(nt), float(h), float(dt)
         self.device = torch.device(device)
         self.K = int(ckpt_every)
@@ -192,7 +196,7 @@ class ShotPlan:
         spec = _lib.TwPlanSpec(order=order, nt=self.nt, h=self.h, dt=self.dt, ckpt_every=self.K,
                                tile_rows=int(tile_rows), graph_steps=int(graph_steps),
                                time_block=int(time_block), forward_only=int(self.forward_only),
-                               tile_cols=int(tile_cols), grad_group=int(grad_group))
+                               tile_cols=int(tile_cols), grad_group=int(grad_group), streams=int(streams))
         self.graph_steps = int(graph_steps)
         self.grad_group = max(1, int(grad_group))
         self.time_block = int(time_block)
@@ -281,6 +285,13 @@ class ShotPlan:
     def backward(self, accumulate=False):
         _lib.check(_lib.lib().tw_plan_backward(self.plan, int(bool(accumulate)),
                                                _lib.stream_of(self.u)), "tw_plan_backward")
+
+    @property
+    def ngroups(self):
+        """Shot groups the C plan runs on separate streams (1 or 2)."""
+        n = _lib.lib().tw_plan_shot_groups(self.plan)
+        _lib.check(min(n, 0), "tw_plan_shot_groups")
+        return n
 
     def set_timing(self, enable):
         _lib.check(_lib.lib().tw_plan_set_timing(self.plan, int(bool(enable))), "tw_plan_set_timing")
