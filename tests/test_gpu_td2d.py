@@ -79,7 +79,7 @@ def test_forward_only_bitexact(order):
     assert np.array_equal(frd.cpu().numpy().reshape(c["nt"], -1), frd_o)
 
 
-@pytest.mark.parametrize("order", [4, 8])
+@pytest.mark.parametrize("order", [2, 4, 8])
 def test_exa_bitexact(order):
     from torchwi_amd.propagator import td2d_exa_cuda
     c = small_case(order, nx=70, ny=33, nt=140, seed=11)
