@@ -127,14 +127,14 @@ typedef struct TwPlanSpec {
                          of one model into one gradient) is currently refused
                          with TW_EINVAL: its kernel faulted on MI355X for a
                          cause not yet found */
-    int streams;      /* time_block > 1: 1 runs all shots on the caller's
-                         stream; 2 runs them as two shot groups of about equal
-                         work, the second on a stream of the plan's own, its
-                         checkpoint segments half a segment behind the first
-                         (the caller's stream waits for both at the end of
-                         each pass); 0 = automatic: 2 whenever the plan
-                         has two shots or more.
-                         Results are the same either way */
+    int streams;      /* 1 runs all shots on the caller's stream; 2 runs
+                         them as two shot groups of about equal work, the
+                         second on a stream of the plan's own, each group
+                         marching through its steps and checkpoint segments
+                         independently (the caller's stream waits for both at
+                         the end of each pass); 0 = automatic: 2 whenever the
+                         plan has two shots or more (grad_group > 1 keeps one
+                         group).  Results are the same either way */
 } TwPlanSpec;
 
 typedef struct TwShotSpec {

@@ -49,8 +49,9 @@ def driver(tmp_path_factory):
     (2, 33, 21, 8, 2, 4, 0, 4, 0, 0), (4, 40, 70, 6, 2, 3, 0, 3, 0, 0),
     # the plan descriptors in the static workspace (tw_plan_create_static)
     (8, 24, 14, 9, 2, 3, 0, 3, 0, 1), (4, 30, 16, 8, 3, 0, 0, 1, 0, 1),
-    # the blocked kernel's two shot groups (static = 2: TW_EMU_STREAMS=2)
-    (8, 24, 14, 10, 3, 3, 0, 3, 0, 2), (4, 40, 22, 9, 5, 0, 0, 3, 0, 2), (2, 33, 21, 13, 2, 6, 0, 2, 0, 2)])
+    # two shot groups on two streams (static = 2: TW_EMU_STREAMS=2), blocked and one step per launch
+    (8, 24, 14, 10, 3, 3, 0, 3, 0, 2), (4, 40, 22, 9, 5, 0, 0, 3, 0, 2), (2, 33, 21, 13, 2, 6, 0, 2, 0, 2),
+    (8, 20, 12, 10, 5, 0, 4, 1, 0, 2), (4, 37, 15, 9, 3, 4, 3, 1, 0, 2), (2, 30, 14, 11, 4, 3, 0, 1, 0, 2)])
 def test_kernel_source_on_host_threads_matches_oracle(driver, tmp_path, order, nx, ny, nt, S, K, graph, tb, rows,
                                                       static):
     rng = np.random.default_rng(order * 7 + nx)

@@ -482,12 +482,16 @@ def test_auto_time_block_large_plan_bitexact():
         assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("order,S,K,nt,windows", [(8, 5, 12, 40, False), (4, 4, 0, 30, False), (8, 6, 9, 28, True)])
-def test_two_shot_groups_on_two_streams_bitexact(order, S, K, nt, windows):
-    """The blocked kernel's two shot groups on two streams (streams = 2; the
-    second group's checkpoint segments half a segment behind the first) give
-    the gathers and per-shot gradients of one stream bit for bit, and those
-    equal the oracle's (shot 1)."""
+@pytest.mark.parametrize("order,S,K,nt,windows,tb,graph", [
+    (8, 5, 12, 40, False, 3, 100), (4, 4, 0, 30, False, 3, 100), (8, 6, 9, 28, True, 3, 100),
+    (8, 5, 12, 40, False, 1, 100), (4, 3, 0, 130, False, 1, 100), (2, 4, 7, 30, False, 1, 0),
+    (8, 6, 9, 28, True, 1, 10)])
+def test_two_shot_groups_on_two_streams_bitexact(order, S, K, nt, windows, tb, graph):
+    """Two shot groups on two streams (streams = 2; each group marching through
+    its steps and checkpoint segments on its own stream), with the blocked
+    kernel and with one step per launch (hipGraph blocks or direct launches),
+    give the gathers and per-shot gradients of one stream bit for bit, and
+    those equal the oracle's (shot 1)."""
     from torchwi_amd import Time2dBatch, Time2dWindows
     c = small_case(order, nx=150, ny=60, nt=nt, seed=S * 7 + K)
     h = c["h"]
@@ -496,19 +500,21 @@ def test_two_shot_groups_on_two_streams_bitexact(order, S, K, nt, windows):
     sx = [(9 + 27 * s + 0.5) * h for s in range(S)]
     sy = [(s % 3 + 0.5) * h for s in range(S)]
     ry = [(1 + s % 2 + 0.5) * h for s in range(S)]
-    print(f"case two shot groups order={order} S={S} K={K} nt={nt} windows={windows}", flush=True)
+    print(f"case two shot groups order={order} S={S} K={K} nt={nt} windows={windows} tb={tb} graph={graph}",
+          flush=True)
     res = []
     for streams in (2, 1):
         if windows:
             wins = [dict(x0=10 * s, x1=10 * s + 90, sx=(20 + s + 0.5) * h, sy=sy[s], ry=ry[s], rx0=3, nrec=80)
                     for s in range(S)]
-            m = Time2dWindows(c["nx"], c["ny"], h, w, c["dt"], order, wins, DEV, ckpt_every=K, time_block=3,
-                              streams=streams)
+            m = Time2dWindows(c["nx"], c["ny"], h, w, c["dt"], order, wins, DEV, ckpt_every=K, time_block=tb,
+                              graph_steps=graph, streams=streams)
             frd = [f.clone() for f in m.modeling(vel)]
             m.misfit_gradient(vel, [0.5 * f for f in frd])
             frd = torch.stack(frd)
         else:
-            m = Time2dBatch(c["nx"], c["ny"], h, w, c["dt"], order, DEV, ckpt_every=K, time_block=3, streams=streams)
+            m = Time2dBatch(c["nx"], c["ny"], h, w, c["dt"], order, DEV, ckpt_every=K, time_block=tb,
+                            graph_steps=graph, streams=streams)
             frd = m.modeling(vel, sx, sy, ry).clone()
             m.misfit_gradient(vel, sx, sy, ry, 0.5 * frd)
         assert m.plan.ngroups == streams

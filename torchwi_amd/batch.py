@@ -95,7 +95,7 @@ class Time2dBatch(torch.nn.Module):
     ``time_block``: time steps per kernel launch; "auto" picks the
     temporally blocked kernel (3 steps) for plans of at least
     shots.AUTO_BLOCK_CELLS cells per step and one step per launch below.
-    ``streams``: the blocked kernel's shot groups (shots.ShotPlan; 0 automatic).
+    ``streams``: shot groups on separate streams (shots.ShotPlan; 0 automatic).
     ``grad_group``: shots per gradient group of the one-step adjoint
     (shots.ShotPlan; values > 1 are refused for now); 1 keeps one gradient
     per shot, whose sum in shot order is the returned gradient.

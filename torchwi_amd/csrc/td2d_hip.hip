@@ -624,8 +624,8 @@ struct Plan {
     int tb = 1;                       // time steps per launch (> 1: temporally blocked kernel)
     int graph_steps = 0;              // steps per captured block (0: direct launches)
     hipStream_t cap = nullptr;        // private stream graphs are captured on
-    Graph g_fwd, g_bwd, g_recomp, g_adj;
-    int* d_ctl = nullptr;             // device control words (block / segment index)
+    Graph g_fwd[2], g_bwd[2], g_recomp[2], g_adj[2];   // per shot group
+    int* d_ctl = nullptr;             // device control words: block / segment index of shot group g at 2g / 2g+1
     int order = 0, nt = 0, K = 0, nseg = 0, nshots = 0, ntiles = 0, per_xcd = 0;
     int ryv = 4;                      // rows per thread of the step kernel
     int cx4 = 16;                     // float4 columns per tile (tile 4*cx4 x (256/cx4)*ryv)
@@ -641,15 +641,15 @@ struct Plan {
     int2* d_tiles = nullptr;
     int2* d_gtiles = nullptr;
     int2* d_vtiles = nullptr;
-    // Blocked plans may run their shots as two groups on two streams (the
-    // caller's and s1), whose launches overlap on the device; group g holds
-    // shots [gs[g], gs[g+1]) and the tile-list ranges [t_off[g], t_off[g+1])
-    // and [vt_off[g], vt_off[g+1]).
+    // A plan may run its shots as two groups on two streams (the caller's
+    // and s1), whose launches overlap on the device; group 0 holds shots
+    // [0, gsplit), group 1 [gsplit, nshots), and group g the tile-list ranges
+    // [t_off[g], t_off[g+1]) and [vt_off[g], vt_off[g+1]).
     int ngroups = 1;
     int gsplit = 0;
     int t_off[3] = {0, 0, 0}, vt_off[3] = {0, 0, 0};
     hipStream_t s1 = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};   // joins: caller -> s1, s1 -> caller
     int static_dev = -1;              // >= 0: the descriptors live in that device's static workspace
     hipStream_t last_stream = nullptr;  // stream of the latest forward / backward pass
     bool used = false;
@@ -658,10 +658,12 @@ struct Plan {
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
         if (s1) (void)hipStreamDestroy(s1);
-        g_fwd.release();
-        g_bwd.release();
-        g_recomp.release();
-        g_adj.release();
+        for (int g = 0; g < 2; ++g) {
+            g_fwd[g].release();
+            g_bwd[g].release();
+            g_recomp[g].release();
+            g_adj[g].release();
+        }
         if (cap) (void)hipStreamDestroy(cap);
     }
 };
@@ -822,15 +824,17 @@ int check_groups(const Plan& p, int tw, int ty)
     return TW_OK;
 }
 
-// Split a blocked plan's shots into two groups of about equal work (tile
-// count), each group's entries contiguous in both work lists, or keep one
-// group.  streams: 1 one group, 2 two, 0 automatic (two whenever there are
-// two shots).  Two groups on two streams overlap one group's memory-bound
-// adjoint with the other's recomputation and fill each launch's tail with the
-// other group's work.  Measured on MI355X, checkpointed gradients, one group /
-// two: 64 shots of 3008 x 1008, nt = 8000: 10.97 / 10.20 s; 16 shots,
-// nt = 3000: 1.266 / 1.165 s; 8 shots: 0.727 / 0.625 s; 256 uneven windows of
-// the 600 x 4000 model, nt = 6000: 10.68 / 9.84 s.
+// Split a plan's shots into two groups of about equal work (tile count),
+// each group's entries contiguous in every work list, or keep one group.
+// streams: 1 one group, 2 two, 0 automatic (two whenever there are two
+// shots).  Two groups on two streams overlap one group's memory-bound adjoint
+// with the other's recomputation and fill each launch's tail with the other
+// group's work.  Measured on MI355X, one group / two: blocked kernel,
+// checkpointed gradients, 64 shots of 3008 x 1008, nt = 8000: 10.97 / 10.20 s;
+// 16 shots, nt = 3000: 1.266 / 1.165 s; 8 shots: 0.727 / 0.625 s; 256 uneven
+// windows of the 600 x 4000 model, nt = 6000: 10.68 / 9.84 s; one step per
+// launch, 32 shots of 392 x 1032, nt = 4000 (as two plans of 16 shots on two
+// streams): full storage 380 / 359 ms, checkpoints every 64 steps 533 / 498 ms.
 int split_groups(Plan& p, int streams)
 {
     const int n = p.ntiles;
@@ -1020,7 +1024,8 @@ int build_plan(const TwPlanSpec* spec, const TwShotSpec* shots, int nshots, Plan
         p.gper_xcd = (p.gntiles + NXCD - 1) / NXCD;
         if ((rc = check_groups(p, tw, ty))) return rc;
     }
-    return TW_OK;
+    // the group adjoint keeps every shot on one stream
+    return split_groups(p, p.ggroup > 1 ? 1 : spec->streams);
 }
 
 size_t plan_bytes(const Plan& p)
@@ -1030,19 +1035,19 @@ size_t plan_bytes(const Plan& p)
 }
 
 template <int NE, unsigned FLAGS, int CX4>
-void launch_batch_nfc(const Plan& p, const StepArgs& a, hipStream_t s)
+void launch_batch_nfc(const Plan& p, const StepArgs& a, hipStream_t s, const int2* tl)
 {
-    const dim3 g(p.per_xcd * NXCD), b(VTHR);
-    if (p.ryv == 4) hipLaunchKernelGGL((k_vstep<NE, FLAGS, 4, CX4>), g, b, 0, s, p.d_desc, p.d_tiles, a);
-    else if (p.ryv == 2) hipLaunchKernelGGL((k_vstep<NE, FLAGS, 2, CX4>), g, b, 0, s, p.d_desc, p.d_tiles, a);
-    else hipLaunchKernelGGL((k_vstep<NE, FLAGS, 1, CX4>), g, b, 0, s, p.d_desc, p.d_tiles, a);
+    const dim3 g(a.per_xcd * NXCD), b(VTHR);
+    if (p.ryv == 4) hipLaunchKernelGGL((k_vstep<NE, FLAGS, 4, CX4>), g, b, 0, s, p.d_desc, tl, a);
+    else if (p.ryv == 2) hipLaunchKernelGGL((k_vstep<NE, FLAGS, 2, CX4>), g, b, 0, s, p.d_desc, tl, a);
+    else hipLaunchKernelGGL((k_vstep<NE, FLAGS, 1, CX4>), g, b, 0, s, p.d_desc, tl, a);
 }
 
 template <int NE, unsigned FLAGS>
-void launch_batch_nf(const Plan& p, const StepArgs& a, hipStream_t s)
+void launch_batch_nf(const Plan& p, const StepArgs& a, hipStream_t s, const int2* tl)
 {
-    if (p.cx4 == 32) launch_batch_nfc<NE, FLAGS, 32>(p, a, s);
-    else launch_batch_nfc<NE, FLAGS, 16>(p, a, s);
+    if (p.cx4 == 32) launch_batch_nfc<NE, FLAGS, 32>(p, a, s, tl);
+    else launch_batch_nfc<NE, FLAGS, 16>(p, a, s, tl);
 }
 
 template <int NE>
@@ -1054,15 +1059,15 @@ void launch_group_ne(const Plan& p, const StepArgs& a, hipStream_t s)
 }
 
 template <int NE>
-int launch_batch_ne(const Plan& p, const StepArgs& a, hipStream_t s)
+int launch_batch_ne(const Plan& p, const StepArgs& a, hipStream_t s, const int2* tl)
 {
     switch (a.flags) {
-    case M_FWD: launch_batch_nf<NE, M_FWD>(p, a, s); break;
-    case M_FWD_VIRT: launch_batch_nf<NE, M_FWD_VIRT>(p, a, s); break;
-    case M_RECOMP: launch_batch_nf<NE, M_RECOMP>(p, a, s); break;
+    case M_FWD: launch_batch_nf<NE, M_FWD>(p, a, s, tl); break;
+    case M_FWD_VIRT: launch_batch_nf<NE, M_FWD_VIRT>(p, a, s, tl); break;
+    case M_RECOMP: launch_batch_nf<NE, M_RECOMP>(p, a, s, tl); break;
     case M_ADJ:
         if (p.ggroup > 1) launch_group_ne<NE>(p, a, s);
-        else launch_batch_nf<NE, M_ADJ>(p, a, s);
+        else launch_batch_nf<NE, M_ADJ>(p, a, s, tl);
         break;
     default: return fail(TW_EINVAL, "internal: no batch step variant for flags %u", a.flags);
     }
@@ -1081,12 +1086,19 @@ int timing_kind(unsigned flags)
     }
 }
 
-// Launch one step kernel.  timed: bracket it with an event pair (direct
-// launches only; inside a capture the caller times the whole graph).
-int launch_batch(Plan& p, StepArgs a, hipStream_t s, bool timed = true)
+// Launch one step kernel over every shot (group < 0) or over shot group
+// `group`'s tiles.  timed: bracket it with an event pair (direct launches
+// only; inside a capture the caller times the whole graph).
+int launch_batch(Plan& p, StepArgs a, hipStream_t s, bool timed = true, int group = -1)
 {
+    const int2* tl = p.d_tiles;
     a.ntiles = p.ntiles;
     a.per_xcd = p.per_xcd;
+    if (group >= 0) {
+        tl += p.t_off[group];
+        a.ntiles = p.t_off[group + 1] - p.t_off[group];
+        a.per_xcd = (a.ntiles + NXCD - 1) / NXCD;
+    }
     if (a.flags == M_ADJ && p.ggroup > 1) {
         a.ntiles = p.gntiles;
         a.per_xcd = p.gper_xcd;
@@ -1097,9 +1109,9 @@ int launch_batch(Plan& p, StepArgs a, hipStream_t s, bool timed = true)
     int rc;
     EventPair* ev = (timed && p.timer.enabled) ? p.timer.next(s) : nullptr;
     if (ev) TW_HIP(hipEventRecord(ev->start, s));
-    if (p.order == 2) rc = launch_batch_ne<1>(p, a, s);
-    else if (p.order == 4) rc = launch_batch_ne<2>(p, a, s);
-    else rc = launch_batch_ne<4>(p, a, s);
+    if (p.order == 2) rc = launch_batch_ne<1>(p, a, s, tl);
+    else if (p.order == 4) rc = launch_batch_ne<2>(p, a, s, tl);
+    else rc = launch_batch_ne<4>(p, a, s, tl);
     if (rc) return rc;
     TW_HIP(hipGetLastError());
     if (ev) {
@@ -1126,43 +1138,46 @@ int set_ctl(const Plan& p, int idx, int op, int v, hipStream_t s)
     return TW_OK;
 }
 
-// Capture the four step-block graphs of a plan (once, on its private stream).
-int batch_prepare_ctl(Plan& p, hipStream_t s);
+// Capture one of the four step-block graphs of a plan or of its shot group
+// `group` (once, on its private stream).  Shot group g keeps its block index
+// in control word 2g and its segment index in 2g + 1 (group < 0: words 0, 1).
+int batch_prepare_ctl(Plan& p, hipStream_t s, int group);
 
-int capture(Plan& p, Graph& gr, int which)
+int capture(Plan& p, Graph& gr, int which, int group)
 {
     if (gr.x) return TW_OK;
     if (!p.cap) TW_HIP(hipStreamCreateWithFlags(&p.cap, hipStreamNonBlocking));
     TW_HIP(hipStreamBeginCapture(p.cap, hipStreamCaptureModeThreadLocal));
     int rc = TW_OK;
+    const int cb = group > 0 ? 2 * group : 0;   // block control word; cb + 1: segment
     StepArgs a;
     std::memset(&a, 0, sizeof a);
-    a.ctl = p.d_ctl;
+    a.ctl = p.d_ctl + cb;
     a.B = p.graph_steps;
-    if (which == 0) {         // forward block: B steps, then ctl0 += 1
+    if (which == 0) {         // forward block: B steps, then block word += 1
         a.phase = PH_FWD_BLK;
         a.flags = (p.K == 0 && !p.fwd_only) ? M_FWD_VIRT : M_FWD;
-        for (int j = 0; j < p.graph_steps && !rc; ++j) { a.j = j; rc = launch_batch(p, a, p.cap, false); }
-        if (!rc) rc = set_ctl(p, 0, 1, 1, p.cap);
-    } else if (which == 1) {  // full-storage adjoint block: B steps, then ctl0 += 1
+        for (int j = 0; j < p.graph_steps && !rc; ++j) { a.j = j; rc = launch_batch(p, a, p.cap, false, group); }
+        if (!rc) rc = set_ctl(p, cb, 1, 1, p.cap);
+    } else if (which == 1) {  // full-storage adjoint block: B steps, then block word += 1
         a.phase = PH_BWD_BLK;
         a.adj = 1;
         a.flags = M_ADJ;
-        for (int j = 0; j < p.graph_steps && !rc; ++j) { a.j = j; rc = launch_batch(p, a, p.cap, false); }
-        if (!rc) rc = set_ctl(p, 0, 1, 1, p.cap);
-    } else if (which == 2) {  // segment recompute: restore from ctl1, K steps
-        rc = batch_prepare_ctl(p, p.cap);
+        for (int j = 0; j < p.graph_steps && !rc; ++j) { a.j = j; rc = launch_batch(p, a, p.cap, false, group); }
+        if (!rc) rc = set_ctl(p, cb, 1, 1, p.cap);
+    } else if (which == 2) {  // segment recompute: restore from the segment word, K steps
+        rc = batch_prepare_ctl(p, p.cap, group);
         a.phase = PH_RECOMP_SEG;
-        a.ctl = p.d_ctl + 1;
+        a.ctl = p.d_ctl + cb + 1;
         a.flags = M_RECOMP;
-        for (int j = 0; j < p.K && !rc; ++j) { a.j = j; rc = launch_batch(p, a, p.cap, false); }
-    } else {                  // segment adjoint: K steps, then ctl1 -= 1
+        for (int j = 0; j < p.K && !rc; ++j) { a.j = j; rc = launch_batch(p, a, p.cap, false, group); }
+    } else {                  // segment adjoint: K steps, then segment word -= 1
         a.phase = PH_ADJ_SEG;
-        a.ctl = p.d_ctl + 1;
+        a.ctl = p.d_ctl + cb + 1;
         a.adj = 1;
         a.flags = M_ADJ;
-        for (int j = 0; j < p.K && !rc; ++j) { a.j = j; rc = launch_batch(p, a, p.cap, false); }
-        if (!rc) rc = set_ctl(p, 1, 1, -1, p.cap);
+        for (int j = 0; j < p.K && !rc; ++j) { a.j = j; rc = launch_batch(p, a, p.cap, false, group); }
+        if (!rc) rc = set_ctl(p, cb + 1, 1, -1, p.cap);
     }
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(p.cap, &g);
@@ -1315,13 +1330,17 @@ int join(Plan& p, int e, hipStream_t s_from, hipStream_t s_to)
     return TW_OK;
 }
 
-// Restore the forward ring for the segment named by control word 1.
-int batch_prepare_ctl(Plan& p, hipStream_t s)
+// Restore the forward ring of every shot (group < 0) or of shot group
+// `group` for the segment named by its segment control word.
+int batch_prepare_ctl(Plan& p, hipStream_t s, int group)
 {
     long long bx = (p.max_field + 255) / 256;
     if (bx > 1024) bx = 1024;
-    hipLaunchKernelGGL(k_batch_prepare, dim3((unsigned)bx, p.nshots), dim3(256), 0, s,
-                       p.d_desc, 0, 0, 0, (const int*)(p.d_ctl + 1), p.K, 0, 0);
+    const int s0 = group == 1 ? p.gsplit : 0;
+    const int ns = group < 0 ? p.nshots : (group == 0 ? p.gsplit : p.nshots - p.gsplit);
+    const int cs = (group > 0 ? 2 * group : 0) + 1;
+    hipLaunchKernelGGL(k_batch_prepare, dim3((unsigned)bx, ns), dim3(256), 0, s,
+                       p.d_desc, 0, 0, 0, (const int*)(p.d_ctl + cs), p.K, 0, s0);
     TW_HIP(hipGetLastError());
     return TW_OK;
 }
@@ -1609,13 +1628,21 @@ int tw_plan_forward(void* plan, void* stream)
         }
         return TW_OK;
     }
+    // one step per launch; with two shot groups, group g on stream sg[g]
+    const bool two = p.ngroups == 2;
+    const int ng = two ? 2 : 1;
+    if (two && ((rc = group_streams(p)) || (rc = join(p, 0, s, p.s1)))) return rc;
+    const hipStream_t sg[2] = {s, p.s1};
     if (p.graph_steps > 0) {
-        if ((rc = capture(p, p.g_fwd, 0))) return rc;
-        if ((rc = set_ctl(p, 0, 0, 0, s))) return rc;
+        for (int g = 0; g < ng; ++g) {
+            if ((rc = capture(p, p.g_fwd[g], 0, two ? g : -1))) return rc;
+            if ((rc = set_ctl(p, 2 * g, 0, 0, sg[g]))) return rc;
+        }
         const int B = p.graph_steps;
         for (int t0 = 0; t0 < p.nt; t0 += B)
-            if ((rc = replay(p, p.g_fwd, kind, (p.nt - t0 < B ? p.nt - t0 : B), s))) return rc;
-        return TW_OK;
+            for (int g = 0; g < ng; ++g)
+                if ((rc = replay(p, p.g_fwd[g], kind, (p.nt - t0 < B ? p.nt - t0 : B), sg[g]))) return rc;
+        return two ? join(p, 1, p.s1, s) : TW_OK;
     }
     StepArgs a;
     std::memset(&a, 0, sizeof a);
@@ -1632,9 +1659,10 @@ int tw_plan_forward(void* plan, void* stream)
             if ((it + 2) % p.K == 0 && (it + 2) / p.K < p.nseg) a.ck0 = (it + 2) / p.K;
             if ((it + 1) % p.K == 0 && (it + 1) / p.K < p.nseg) a.ck1 = (it + 1) / p.K;
         }
-        if ((rc = launch_batch(p, a, s))) return rc;
+        for (int g = 0; g < ng; ++g)
+            if ((rc = launch_batch(p, a, sg[g], true, two ? g : -1))) return rc;
     }
-    return TW_OK;
+    return two ? join(p, 1, p.s1, s) : TW_OK;
 }
 
 int tw_plan_backward(void* plan, int accumulate, void* stream)
@@ -1668,10 +1696,11 @@ int tw_plan_backward(void* plan, int accumulate, void* stream)
             return two ? join(p, 1, p.s1, s) : TW_OK;
         }
         // Checkpoint segments last to first, each a recomputation then the
-        // adjoint.  With two groups the second lags the first by half a
-        // segment: group 1 recomputes segment seg while group 0 runs its
-        // adjoint, and group 0 recomputes seg-1 while group 1 runs its
-        // adjoint (events 2 and 3 order the recomputations).
+        // adjoint; the two groups' streams march through their segments
+        // independently, so one group's memory-bound adjoint overlaps the
+        // other's recomputation wherever the two drift apart.  Measured on
+        // MI355X (64 shots of 3008 x 1008, nt = 1800): 2.284 s, against
+        // 2.318 s with the groups held in lockstep half a segment apart.
         int ab = 0;
         for (int seg = p.nseg - 1; seg >= 0; --seg) {
             const int t0 = seg * p.K;
@@ -1679,15 +1708,12 @@ int tw_plan_backward(void* plan, int accumulate, void* stream)
             for (int g = 0; g < ng; ++g) {
                 hipStream_t sg = g == 0 ? s : p.s1;
                 const int grp = two ? g : -1;
-                if (two && g == 0 && seg < p.nseg - 1) TW_HIP(hipStreamWaitEvent(s, p.ev[3], 0));
-                if (two && g == 1) TW_HIP(hipStreamWaitEvent(p.s1, p.ev[2], 0));
                 if ((rc = batch_prepare(p, 0, seg, t0, sg, grp))) return rc;
                 int b = 0;
                 for (int t = t0; t < t1; t += p.tb, ++b) {
                     const int n = t1 - t < p.tb ? t1 - t : p.tb;
                     if ((rc = launch_block(p, M_RECOMP, t, +1, n, b & 1, t - t0, 0, sg, grp))) return rc;
                 }
-                if (two) TW_HIP(hipEventRecord(p.ev[g == 0 ? 2 : 3], sg));
                 int a2 = ab;
                 for (int it = t1 - 1; it >= t0; it -= p.tb, ++a2) {
                     const int n = it - t0 + 1 < p.tb ? it - t0 + 1 : p.tb;
@@ -1698,58 +1724,74 @@ int tw_plan_backward(void* plan, int accumulate, void* stream)
         }
         return two ? join(p, 1, p.s1, s) : TW_OK;
     }
+    // one step per launch; with two shot groups, group g on stream sg[g],
+    // each group's segments independent of the other's
+    const bool two = p.ngroups == 2;
+    const int ng = two ? 2 : 1;
+    if (two && ((rc = group_streams(p)) || (rc = join(p, 0, s, p.s1)))) return rc;
+    const hipStream_t sg[2] = {s, p.s1};
     if (p.graph_steps > 0) {
         if (p.K == 0) {
-            if ((rc = capture(p, p.g_bwd, 1))) return rc;
-            if ((rc = set_ctl(p, 0, 0, 0, s))) return rc;
+            for (int g = 0; g < ng; ++g) {
+                if ((rc = capture(p, p.g_bwd[g], 1, two ? g : -1))) return rc;
+                if ((rc = set_ctl(p, 2 * g, 0, 0, sg[g]))) return rc;
+            }
             const int B = p.graph_steps;
             for (int t0 = 0; t0 < p.nt; t0 += B)
-                if ((rc = replay(p, p.g_bwd, TW_KIND_ADJ, (p.nt - t0 < B ? p.nt - t0 : B), s))) return rc;
-            return TW_OK;
+                for (int g = 0; g < ng; ++g)
+                    if ((rc = replay(p, p.g_bwd[g], TW_KIND_ADJ, (p.nt - t0 < B ? p.nt - t0 : B), sg[g]))) return rc;
+            return two ? join(p, 1, p.s1, s) : TW_OK;
         }
-        if ((rc = capture(p, p.g_recomp, 2))) return rc;
-        if ((rc = capture(p, p.g_adj, 3))) return rc;
-        if ((rc = set_ctl(p, 1, 0, p.nseg - 1, s))) return rc;
+        for (int g = 0; g < ng; ++g) {
+            if ((rc = capture(p, p.g_recomp[g], 2, two ? g : -1))) return rc;
+            if ((rc = capture(p, p.g_adj[g], 3, two ? g : -1))) return rc;
+            if ((rc = set_ctl(p, 2 * g + 1, 0, p.nseg - 1, sg[g]))) return rc;
+        }
         for (int seg = p.nseg - 1; seg >= 0; --seg) {
             const int t0 = seg * p.K;
             const int n = (t0 + p.K < p.nt ? p.K : p.nt - t0);
-            if ((rc = replay(p, p.g_recomp, TW_KIND_RECOMP, n, s))) return rc;
-            if ((rc = replay(p, p.g_adj, TW_KIND_ADJ, n, s))) return rc;
+            for (int g = 0; g < ng; ++g) {
+                if ((rc = replay(p, p.g_recomp[g], TW_KIND_RECOMP, n, sg[g]))) return rc;
+                if ((rc = replay(p, p.g_adj[g], TW_KIND_ADJ, n, sg[g]))) return rc;
+            }
         }
-        return TW_OK;
+        return two ? join(p, 1, p.s1, s) : TW_OK;
     }
     StepArgs a;
     std::memset(&a, 0, sizeof a);
     a.ck0 = a.ck1 = -1;
-    auto adj_step = [&](int it, int slot) {
+    auto adj_step = [&](int it, int slot, int g) {
         a.it = it;
         a.ring = p.nt - 1 - it;
         a.adj = 1;
         a.flags = M_ADJ;
         a.vslot = slot;
-        return launch_batch(p, a, s);
+        return launch_batch(p, a, sg[g], true, two ? g : -1);
     };
     if (p.K == 0) {
         for (int it = p.nt - 1; it >= 0; --it)
-            if ((rc = adj_step(it, it))) return rc;
-        return TW_OK;
+            for (int g = 0; g < ng; ++g)
+                if ((rc = adj_step(it, it, g))) return rc;
+        return two ? join(p, 1, p.s1, s) : TW_OK;
     }
     for (int seg = p.nseg - 1; seg >= 0; --seg) {
         const int t0 = seg * p.K;
         const int t1 = (t0 + p.K < p.nt) ? t0 + p.K : p.nt;
-        if ((rc = batch_prepare(p, 0, seg, t0, s))) return rc;
-        for (int t = t0; t < t1; ++t) {
-            a.it = t;
-            a.ring = t;
-            a.adj = 0;
-            a.flags = M_RECOMP;
-            a.vslot = t - t0;
-            if ((rc = launch_batch(p, a, s))) return rc;
+        for (int g = 0; g < ng; ++g) {
+            if ((rc = batch_prepare(p, 0, seg, t0, sg[g], two ? g : -1))) return rc;
+            for (int t = t0; t < t1; ++t) {
+                a.it = t;
+                a.ring = t;
+                a.adj = 0;
+                a.flags = M_RECOMP;
+                a.vslot = t - t0;
+                if ((rc = launch_batch(p, a, sg[g], true, two ? g : -1))) return rc;
+            }
+            for (int it = t1 - 1; it >= t0; --it)
+                if ((rc = adj_step(it, it - t0, g))) return rc;
         }
-        for (int it = t1 - 1; it >= t0; --it)
-            if ((rc = adj_step(it, it - t0))) return rc;
     }
-    return TW_OK;
+    return two ? join(p, 1, p.s1, s) : TW_OK;
 }
 
 void tw_plan_destroy(void* plan)

@@ -128,10 +128,10 @@ class ShotPlan:
             gradient, the group's first shot's grad_of) is refused by the
             library for now: its adjoint kernel faulted on MI355X for a cause
             not yet found.
-    streams: blocked plans only: 1 all shots on the caller's stream, 2 two
-            shot groups on two streams whose launches overlap (the second group
-            half a checkpoint segment behind), 0 automatic (two whenever there
-            are two shots).  The results are the same.
+    streams: 1 all shots on the caller's stream, 2 two shot groups on two
+            streams whose launches overlap (each group marching through its
+            steps and checkpoint segments independently), 0 automatic (two
+            whenever there are two shots).  The results are the same.
     static_workspace: put the plan's descriptors in the library's static
             per-device workspace (tw_plan_create_static: no device allocation
             by the library; one such plan per device at a time, graph_steps
