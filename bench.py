@@ -1,7 +1,7 @@
 """Benchmark: checkpointed batched forward + adjoint gradient of the
 time-domain 2D acoustic propagator on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nt NT] [--compare 0|1] ...
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nt NT] [--compare 0|1|LEGS] ...
 
 Headline workload (one "step" = one misfit gradient over every shot): 64
 ricker(6) shots on the 1000 x 3000 dipping-layer model (padded 1008 x 3008,
@@ -68,8 +68,14 @@ def parse():
                     help="blocked kernel: 1 one stream, 2 two shot groups on two streams, 0 automatic")
     ap.add_argument("--no-timing", action="store_true",
                     help="skip the extra gradient timed per launch (the kernels' roofline)")
-    ap.add_argument("--compare", type=int, default=1,
-                    help="also time the 384 x 1024 x 32-shot gradient (full storage, checkpointed, plain form)")
+    ap.add_argument("--compare", default="1",
+                    help="also time the 384 x 1024 x 32-shot gradient: 1 every leg (full storage, checkpointed, "
+                         "plain form), 0 none, or a comma list of full, ckpt, plain")
+    ap.add_argument("--compare-nt", type=int, default=0, help="nt of the 384 x 1024 legs (0: the workload's 4000)")
+    ap.add_argument("--compare-streams", type=int, default=0,
+                    help="shot groups of the 384 x 1024 batched legs (0 automatic, 1 one stream, 2 two groups)")
+    ap.add_argument("--headline", type=int, default=1,
+                    help="0: skip the headline gradient (profiling runs of the other legs; value is then null)")
     ap.add_argument("--serving-requests", type=int, default=12,
                     help="requests of the serving-stream leg (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -248,14 +254,14 @@ def run_scale(args, dev):
     return r
 
 
-def run_batched(c, K, steps, warmup, dev, time_block=1):
+def run_batched(c, K, steps, warmup, dev, time_block=1, streams=0):
     """Time `steps` misfit gradients of the batched engine on workload c."""
     import torch
     from torchwi_amd import Time2dBatch
     order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
     S = len(c["sx"])
     model = Time2dBatch(nx, ny, h, torch.from_numpy(c["w"]), dt, order, dev, ckpt_every=K,
-                        time_block=time_block)
+                        time_block=time_block, streams=streams)
     vel = torch.from_numpy(c["vel"]).to(dev)
     rng = np.random.default_rng(7)
     vtrue = torch.from_numpy(c["vel"] * (1.0 + 0.02 * rng.standard_normal(c["vel"].shape)).astype(np.float32)).to(dev)
@@ -270,7 +276,7 @@ def run_batched(c, K, steps, warmup, dev, time_block=1):
     elapsed = time.perf_counter() - t0
     mg = model.plan.models[0]
     cells = 2.0 * S * mg.dimx * mg.dimy * nt * steps
-    r = {"ckpt_every": model.plan.K, "time_block": model.plan.time_block,
+    r = {"ckpt_every": model.plan.K, "time_block": model.plan.time_block, "shot_groups": model.plan.ngroups,
          "value": round(cells / elapsed / 1e9, 3), "ms_per_gradient": round(elapsed / steps * 1e3, 3),
          "gradient_shots_per_s": round(S * steps / elapsed, 3),
          "effective_hbm_frac": round(cells / elapsed * (BYTES_GRAD_STEP / 2.0) / (HBM_PEAK_GBS * 1e9), 4)}
@@ -307,21 +313,27 @@ def run_plain(c, nshots, dev):
     return {"gradient_shots_per_s": round(nshots / el, 3), "shots_timed": nshots, "s_per_shot": round(el / nshots, 4)}
 
 
-def run_compare(dev):
+def run_compare(dev, legs=("full", "ckpt", "plain"), nt=0, streams=0):
     """The 384 x 1024 model, 32 ricker(8) shots, nt = 4000: batched gradient
     with full storage (it fits: 212 GB) and checkpointed, against the plain
     one-shot-per-launch form."""
     from torchwi_amd import workloads as W
     from torchwi_amd.shots import default_ckpt_interval
-    c = W.dipping_384x1024()
-    full = run_batched(c, 0, 2, 1, dev)
-    ck = run_batched(c, default_ckpt_interval(c["nt"]), 1, 0, dev)
-    pl = run_plain(c, 2, dev)
-    return {"workload": "32 ricker(8) shots, 384x1024 model (padded 392x1032), h=12.5 m, dt=1 ms, nt=4000, order 8, "
-                        "one step per launch in hipGraph blocks of 100 steps",
-            "full_storage": full, "checkpointed": ck, "plain_one_shot": pl,
-            "speedup_vs_plain": {"full_storage": round(full["gradient_shots_per_s"] / pl["gradient_shots_per_s"], 2),
-                                 "checkpointed": round(ck["gradient_shots_per_s"] / pl["gradient_shots_per_s"], 2)}}
+    c = W.dipping_384x1024(nt=nt) if nt else W.dipping_384x1024()
+    out = {"workload": "32 ricker(8) shots, 384x1024 model (padded 392x1032), h=12.5 m, dt=1 ms, nt=%d, order 8, "
+                       "one step per launch in hipGraph blocks of 100 steps" % c["nt"]}
+    if "full" in legs:
+        out["full_storage"] = run_batched(c, 0, 2, 1, dev, streams=streams)
+    if "ckpt" in legs:
+        out["checkpointed"] = run_batched(c, default_ckpt_interval(c["nt"]), 1, 0, dev, streams=streams)
+    if "plain" in legs:
+        pl = out["plain_one_shot"] = run_plain(c, 2, dev)
+        sp = {}
+        for name, key in (("full_storage", "full_storage"), ("checkpointed", "checkpointed")):
+            if key in out:
+                sp[name] = round(out[key]["gradient_shots_per_s"] / pl["gradient_shots_per_s"], 2)
+        out["speedup_vs_plain"] = sp
+    return out
 
 
 def run_serving(n, dev):
@@ -364,6 +376,21 @@ def main():
 
     dev = torch.device("cuda:0")
     probe = copy_probe(args.copy_probe) if args.copy_probe else None
+    legs = {"1": ("full", "ckpt", "plain"), "0": ()}.get(args.compare, tuple(args.compare.split(",")))
+    if any(x not in ("full", "ckpt", "plain") for x in legs):
+        print("--compare takes 0, 1 or a comma list of full, ckpt, plain (got %r)" % args.compare, file=sys.stderr)
+        sys.exit(2)
+    if not args.headline:
+        out = {"metric": METRIC, "value": None, "unit": "Gcell-updates/s", "n_gpus": 1,
+               "note": "headline gradient skipped (--headline 0)"}
+        if legs:
+            out["batched_384x1024"] = run_compare(dev, legs, args.compare_nt, args.compare_streams)
+        if args.serving_requests:
+            out["serving"] = run_serving(args.serving_requests, dev)
+        if probe:
+            out["copy_probe"] = probe
+        print(json.dumps(out), flush=True)
+        return
     r = run_scale(args, dev)
     S, T, elapsed, lat = r["S"], r["T"], r["elapsed"], r["lat"]
     value = r["cells_step"] * args.steps / elapsed / 1e9
@@ -405,8 +432,8 @@ def main():
             out["roofline"]["traffic"] = t["total_bytes"]
             out["roofline"]["traffic_bytes_per_cell"] = t["bytes_per_cell"]
             out["roofline"]["traffic_GBs"] = round(t["total_bytes"] / (per[dom]["avg_us_per_launch"] * 1e-6) / 1e9, 1)
-    if args.compare:
-        out["batched_384x1024"] = run_compare(dev)
+    if legs:
+        out["batched_384x1024"] = run_compare(dev, legs, args.compare_nt, args.compare_streams)
     if args.serving_requests:
         out["serving"] = run_serving(args.serving_requests, dev)
     if probe:

@@ -1,6 +1,6 @@
 """Reduce rocprofv3 counter dumps to HBM bytes per launch of each step kernel.
 
-    python -m torchwi_amd.profiling FETCH_CSV WRITE_CSV COPY_MIB CELLS_PER_LAUNCH OUT.json WORKLOAD
+    python -m torchwi_amd.profiling FETCH_CSV WRITE_CSV COPY_MIB CELLS_PER_LAUNCH OUT.json WORKLOAD [KEY]
 
 FETCH_CSV / WRITE_CSV are the ``*_counter_collection.csv`` files of two
 ``rocprofv3 --pmc FETCH_SIZE`` and ``--pmc WRITE_SIZE`` runs of the same
@@ -12,7 +12,8 @@ step-flag template argument of their names (3 forward, 7 forward with
 virtual-source storage, 5 recomputation, 24 adjoint); the temporally blocked
 kernel (``k_pipe<NE, FLAGS, T>``) is compared with its own compulsory bytes
 per launch, the one-step kernel (``k_vstep``) with the per-unit bytes of one
-cell-update.
+cell-update.  With KEY, the kernels go under that key of OUT.json (which may
+already hold other keys, e.g. one per storage mode of the same workload).
 """
 import csv
 import json
@@ -103,12 +104,23 @@ def summarize(fetch_csv, write_csv, copy_mib, cells_per_launch, workload):
 
 def main(argv=None):
     a = sys.argv[1:] if argv is None else argv
-    if len(a) != 6:
+    if len(a) not in (6, 7):
         print(__doc__.strip().splitlines()[2], file=sys.stderr)
         return 2
     out = summarize(a[0], a[1], int(a[2]), int(a[3]), a[5])
+    if len(a) == 7:
+        try:
+            with open(a[4]) as f:
+                merged = json.load(f)
+        except (OSError, ValueError):
+            merged = {}
+        merged.update({k: v for k, v in out.items() if k != "kernels"})
+        merged[a[6]] = out["kernels"]
+        out_doc = merged
+    else:
+        out_doc = out
     with open(a[4], "w") as f:
-        json.dump(out, f, indent=1)
+        json.dump(out_doc, f, indent=1)
     print(json.dumps(out["kernels"]))
     return 0
 
