@@ -120,7 +120,8 @@ def copy_probe(mb, reps=10):
 
 
 def cpu_baseline():
-    """Config-1 forward + gradient on the host cores: the reference's own CPU
+    """Forward + gradient of the three-layer 100 x 200 model (20-cell pad) on
+    the host cores: the reference's own CPU
     path when it was built (oracle/_ref), else the oracle's restatement."""
     from oracle import oracle as O
     from torchwi_amd import workloads as W
@@ -337,7 +338,8 @@ def run_compare(dev, legs=("full", "ckpt", "plain"), nt=0, streams=0):
 
 
 def run_serving(n, dev):
-    """A short serving stream on the config-5 grids: n Poisson requests at
+    """A short serving stream on the four serving grid classes (256 x 512 to
+    1200 x 3600): n Poisson requests at
     80% of the measured back-to-back capacity; p50/p99 latency."""
     import torch
     from torchwi_amd import serving as SV

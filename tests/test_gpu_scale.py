@@ -1,4 +1,4 @@
-"""GPU checks at the configurations' own full sizes, against the oracle
+"""GPU checks at the benchmark workloads' own full sizes, against the oracle
 directly where the oracle finishes in seconds to a minute on the host cores,
 and through exact properties elsewhere.
 
@@ -25,7 +25,7 @@ pytestmark = pytest.mark.gpu
 DEV = "cuda"
 
 
-def test_config2_shot0_gather_and_gradient_equal_oracle():
+def test_384x1024_32_shots_shot0_gather_and_gradient_equal_oracle():
     from torchwi_amd import Time2dBatch
     from torchwi_amd import workloads as W
     c = W.dipping_384x1024()
@@ -56,7 +56,7 @@ def test_config2_shot0_gather_and_gradient_equal_oracle():
     frd64, virt64 = O.forward64(vpad, c["w"], *a, c["sx"][0], c["sy"][0], c["ry"][0])
     grad64 = O.backward64(virt64, vpad, resid0, *a, c["ry"][0])
     del virt64
-    record_drift("config 2: 384x1024, order 8, nt=4000, shot 0 of 32", frd=rel_l2(frd0, frd64),
+    record_drift("384x1024, order 8, nt=4000, shot 0 of 32", frd=rel_l2(frd0, frd64),
                  grad=rel_l2(grad0, grad64))
     assert rel_l2(frd0, frd64) < 1e-4 and rel_l2(grad0, grad64) < 1e-3
 
@@ -69,7 +69,7 @@ def _misfit64(frd, d_obs):
     return J
 
 
-def test_config3_64_shots_blocked_equals_one_step_oracle_gather_and_finite_differences():
+def test_1000x3000_64_shots_blocked_equals_one_step_oracle_gather_and_finite_differences():
     """Central differences along 5 seeded smooth directions d_k (seeds
     10-14: white noise, Gaussian sigma = 20 cells, unit max-norm, zero
     within 10 cells of the left, right and bottom edges), eps = 1e-3:
@@ -103,11 +103,11 @@ def test_config3_64_shots_blocked_equals_one_step_oracle_gather_and_finite_diffe
     vtrue = vel * (1.0 + 0.02 * torch.from_numpy(direction(nx, ny, seed=99, sigma=30.0)).to(DEV))
     pos = (c["sx"], c["sy"], c["ry"])
 
-    print("case config 3: 64 shots 1000x3000 nt=8000 K=90, blocked kernel: observed data", flush=True)
+    print("case 1000x3000 64 shots: 64 shots 1000x3000 nt=8000 K=90, blocked kernel: observed data", flush=True)
     m = Time2dBatch(nx, ny, h, w, dt, order, DEV, ckpt_every=K, time_block=3)
     d_obs = m.modeling(vtrue, *pos).clone()
     assert m.plan.time_block == 3 and m.plan.K == K
-    print("case config 3: blocked checkpointed gradient", flush=True)
+    print("case 1000x3000 64 shots: blocked checkpointed gradient", flush=True)
     loss_b, g = m.misfit_gradient(vel, *pos, d_obs)
     loss_b, g = float(loss_b), g.clone()
     frd_b = m.plan.frd_all().clone()
@@ -121,7 +121,7 @@ def test_config3_64_shots_blocked_equals_one_step_oracle_gather_and_finite_diffe
         dvec = torch.from_numpy(direction(nx, ny, seed=10 + k)).to(DEV)
         J = {}
         for e in (eps, -eps, 2 * eps, -2 * eps):
-            print(f"case config 3: direction seed {10 + k}, eps {e:+g}", flush=True)
+            print(f"case 1000x3000 64 shots: direction seed {10 + k}, eps {e:+g}", flush=True)
             J[e] = _misfit64(m.modeling(vel * (1.0 + e * dvec), *pos), d_obs)
         fd1 = (J[eps] - J[-eps]) / (2 * eps)
         fd2 = (J[2 * eps] - J[-2 * eps]) / (4 * eps)
@@ -130,7 +130,7 @@ def test_config3_64_shots_blocked_equals_one_step_oracle_gather_and_finite_diffe
     del m
     torch.cuda.empty_cache()
 
-    print("case config 3: one-step checkpointed gradient", flush=True)
+    print("case 1000x3000 64 shots: one-step checkpointed gradient", flush=True)
     m1 = Time2dBatch(nx, ny, h, w, dt, order, DEV, ckpt_every=K, time_block=1)
     loss_1, g1 = m1.misfit_gradient(vel, *pos, d_obs)
     assert m1.plan.time_block == 1
@@ -142,7 +142,7 @@ def test_config3_64_shots_blocked_equals_one_step_oracle_gather_and_finite_diffe
     del m1, grads_b, d_obs, g1
     torch.cuda.empty_cache()
 
-    print("case config 3: shot 0 gather, oracle forward-only FP32 and FP64", flush=True)
+    print("case 1000x3000 64 shots: shot 0 gather, oracle forward-only FP32 and FP64", flush=True)
     vpad = O.pad_vel(c["vel"], order)
     a = (order, nx + order, ny + order, h, dt, c["sx"][0], c["sy"][0], c["ry"][0])
     f0 = frd_b[0].cpu().numpy()
@@ -150,13 +150,13 @@ def test_config3_64_shots_blocked_equals_one_step_oracle_gather_and_finite_diffe
     assert np.array_equal(f0, frd_o), "shot 0 gather vs oracle"
     frd64, _ = O.forward64(vpad, c["w"], *a, store_virt=False)
     gamma0 = rel_l2(f0, frd64)
-    record_drift("config 3: 1000x3000, order 8, nt=8000, shot 0 of 64", frd=gamma0)
+    record_drift("1000x3000, order 8, nt=8000, shot 0 of 64", frd=gamma0)
 
     bound = 3.0 * max(gamma0, 1e-7) * rn * dn / eps
     for seed, fd1, fdr, an in fds:
-        print(f"config 3 FD seed {seed}: adjoint {an:.6e} FD {fd1:.6e} (rel {abs(fd1 - an) / abs(an):.2e}) "
+        print(f"1000x3000 FD seed {seed}: adjoint {an:.6e} FD {fd1:.6e} (rel {abs(fd1 - an) / abs(an):.2e}) "
               f"Richardson {fdr:.6e} (|diff| {abs(fdr - an):.3e}, bound {bound:.3e})", flush=True)
-        record_drift(f"config 3: central differences, direction seed {seed}, eps 1e-3", adjoint=an, fd=fd1,
+        record_drift(f"1000x3000, 64 shots: central differences, direction seed {seed}, eps 1e-3", adjoint=an, fd=fd1,
                      fd_richardson=fdr, fd_rel_err=abs(fd1 - an) / abs(an),
                      richardson_rel_err=abs(fdr - an) / abs(an), noise_bound_abs=bound)
     scale = max(abs(an) for _, _, _, an in fds)

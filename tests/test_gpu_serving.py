@@ -130,7 +130,7 @@ def test_failing_request_is_isolated():
         _check_same(alone, res[i])
 
 
-def test_config5_grid_classes_alone_equal_mix():
+def test_serving_grid_classes_alone_equal_mix():
     """One request of each serving grid class (256x512, 512x1024, 800x2400,
     1200x3600 at h = 10 m, dt = 1 ms) with nt >= 2000, plus a second request
     sharing a key, served as one stream: each result is bit-identical to the
@@ -143,9 +143,9 @@ def test_config5_grid_classes_alone_equal_mix():
     mk = lambda i: make_request(spec[i][0], *spec[i][1:], seed=100 + i, device=DEV, rid=i)  # noqa: E731
     srv = Server(DEV, slots=4)
     mixed = srv.run([mk(i) for i in range(len(spec))], realtime=False)
-    print(f"config 5 mix: max concurrent {srv.max_concurrent}, largest batch {srv.max_batch_requests} requests",
+    print(f"serving grid classes mix: max concurrent {srv.max_concurrent}, largest batch {srv.max_batch_requests} requests",
           flush=True)
     for i in range(len(spec)):
-        print(f"case config 5 request={i} {spec[i]} alone", flush=True)
+        print(f"case serving grid classes request={i} {spec[i]} alone", flush=True)
         alone = Server(DEV, slots=4, merge=False).run([mk(i)], realtime=False)[0]
         _check_same(alone, mixed[i])

@@ -206,7 +206,7 @@ def test_three_layer_100x200_forward_gradient_bitexact():
     loss.backward()
     f, g = frd.detach().cpu().numpy(), vel.grad.cpu().numpy()
     print(f"three-layer 100x200: rel-L2 vs oracle frd {rel_l2(f, frd_o):.2e} grad {rel_l2(g, grad_o):.2e}")
-    record_drift("config 1: three-layer 100x200 + 20-cell pad, order 8, nt=1000", frd=rel_l2(f, frd64),
+    record_drift("three-layer 100x200 + 20-cell pad, order 8, nt=1000", frd=rel_l2(f, frd64),
                  grad=rel_l2(g, grad64))
     assert np.array_equal(f, frd_o)
     assert np.array_equal(g, grad_o)
