@@ -119,40 +119,66 @@ def copy_probe(mb, reps=10):
     return out
 
 
-def cpu_baseline():
-    """Forward + gradient of the three-layer 100 x 200 model (20-cell pad) on
-    the host cores: the reference's own CPU
-    path when it was built (oracle/_ref), else the oracle's restatement."""
+CPU_SAMPLE_STEPS = 1200    # time steps of the headline workload's shot 0 timed on the host cores
+
+
+def _cpu_run(c, impl, nt_sample=None, d_obs_model=None):
+    """One forward (full virtual-source storage) + adjoint gradient of shot 0
+    of workload c on the host cores (the reference's own CPU path, impl
+    "ref", or the oracle's restatement, "port"); returns (seconds per run of
+    the median of three after one warm-up, credited cell-updates per run)."""
     from oracle import oracle as O
-    from torchwi_amd import workloads as W
-    cores = len(os.sched_getaffinity(0))
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    c = W.three_layer_100x200()
-    order, nx, ny, nt, h, dt = c["order"], c["nx"], c["ny"], c["nt"], c["h"], c["dt"]
+    order, nx, ny, h, dt = c["order"], c["nx"], c["ny"], c["h"], c["dt"]
+    w = c["w"] if nt_sample is None else c["w"][:nt_sample]
+    nt = len(w)
     dimx, dimy = nx + order, ny + order
     vpad = O.pad_vel(c["vel"], order)
-    d_obs, _ = O.forward(O.pad_vel(c["vel_true"], order), c["w"], order, dimx, dimy, h, dt,
-                         c["sx"][0], c["sy"][0], c["ry"][0], store_virt=False)
-    impl = "ref" if O.ref_available() else "oracle"
+    if d_obs_model is not None:
+        d_obs, _ = O.forward(O.pad_vel(d_obs_model, order), w, order, dimx, dimy, h, dt,
+                             c["sx"][0], c["sy"][0], c["ry"][0], store_virt=False)
+    else:
+        d_obs = np.zeros((nt, nx), np.float32)
 
     def run():
-        frd, virt = O.forward(vpad, c["w"], order, dimx, dimy, h, dt, c["sx"][0], c["sy"][0],
-                              c["ry"][0], impl=impl)
+        frd, virt = O.forward(vpad, w, order, dimx, dimy, h, dt, c["sx"][0], c["sy"][0], c["ry"][0], impl=impl)
         O.backward(virt, vpad, frd - d_obs, order, dimx, dimy, h, dt, c["ry"][0], impl=impl)
 
     run()
     ts = []
-    for _ in range(5):
+    for _ in range(3):
         t0 = time.perf_counter()
         run()
         ts.append(time.perf_counter() - t0)
-    t = statistics.median(ts)
-    cells = 2.0 * dimx * dimy * nt
+    return statistics.median(ts), 2.0 * dimx * dimy * nt
+
+
+def cpu_baseline():
+    """The reference's own CPU path (oracle/_ref, OpenMP on every host core;
+    the oracle's restatement where it was not built) on a bounded sample of
+    the headline workload: shot 0 of the 1000 x 3000 model, its first
+    CPU_SAMPLE_STEPS time steps, forward with full virtual-source storage
+    (the reference's CPU path has no checkpointing) + adjoint gradient.  Also
+    the three-layer 100 x 200 model (20-cell pad, nt = 1000), whose grid fits
+    in the host caches."""
+    from oracle import oracle as O
+    from torchwi_amd import workloads as W
+    cores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    impl = "ref" if O.ref_available() else "oracle"
+    c = W.dipping_1000x3000(nshots=64)
+    t, cells = _cpu_run(c, impl, nt_sample=CPU_SAMPLE_STEPS)
+    small = W.three_layer_100x200()
+    ts, cells_s = _cpu_run(small, impl, d_obs_model=small["vel_true"])
     return {"value": round(cells / t / 1e9, 4), "unit": "Gcell-updates/s",
             "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
             "kind": "reference" if impl == "ref" else "port",
-            "sample": "100x200 model + 20-cell pad (248x128 padded, order 8), nt=1000, 1 shot, "
-                      "forward with full storage + adjoint gradient; median of 5 runs (%.3f s each)" % t}
+            "sample": "shot 0 of the 1000x3000 model (1008x3008 padded, order 8), its first %d of 8000 time steps, "
+                      "forward with full storage + adjoint gradient; median of 3 runs after one warm-up "
+                      "(%.2f s each, %.1f s of CPU work timed)" % (CPU_SAMPLE_STEPS, t, 4 * t),
+            "small_model": {"value": round(cells_s / ts / 1e9, 4),
+                            "sample": "100x200 model + 20-cell pad (248x128 padded, order 8), nt=1000, 1 shot, "
+                                      "forward with full storage + adjoint gradient; median of 3 runs (%.3f s each)"
+                                      % ts}}
 
 
 def kernel_table(tm, cells, T):
