@@ -119,19 +119,25 @@ def copy_probe(mb, reps=10):
     return out
 
 
-CPU_SAMPLE_STEPS = 1200    # time steps of the headline workload's shot 0 timed on the host cores
+# time steps of the headline workload's shot 0 timed on the host cores: the reference's CPU path indexes
+# its virtual sources with int (propagator_cpu.cpp:71,144), so nt * dimx * dimy must stay below 2^31
+# (708 steps of the 3008 x 1008 padded grid)
+CPU_SAMPLE_STEPS = 700
 
 
 def _cpu_run(c, impl, nt_sample=None, d_obs_model=None):
     """One forward (full virtual-source storage) + adjoint gradient of shot 0
     of workload c on the host cores (the reference's own CPU path, impl
-    "ref", or the oracle's restatement, "port"); returns (seconds per run of
-    the median of three after one warm-up, credited cell-updates per run)."""
+    "ref", or the oracle's restatement, "port"); returns (seconds per run,
+    the median of five after one warm-up, credited cell-updates per run)."""
     from oracle import oracle as O
     order, nx, ny, h, dt = c["order"], c["nx"], c["ny"], c["h"], c["dt"]
     w = c["w"] if nt_sample is None else c["w"][:nt_sample]
     nt = len(w)
     dimx, dimy = nx + order, ny + order
+    if impl == "ref" and nt * dimx * dimy >= 2 ** 31:
+        raise ValueError("the reference's CPU path indexes virt with int: nt * dimx * dimy = %d >= 2^31"
+                         % (nt * dimx * dimy))
     vpad = O.pad_vel(c["vel"], order)
     if d_obs_model is not None:
         d_obs, _ = O.forward(O.pad_vel(d_obs_model, order), w, order, dimx, dimy, h, dt,
@@ -145,7 +151,7 @@ def _cpu_run(c, impl, nt_sample=None, d_obs_model=None):
 
     run()
     ts = []
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         run()
         ts.append(time.perf_counter() - t0)
@@ -173,11 +179,11 @@ def cpu_baseline():
             "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
             "kind": "reference" if impl == "ref" else "port",
             "sample": "shot 0 of the 1000x3000 model (1008x3008 padded, order 8), its first %d of 8000 time steps, "
-                      "forward with full storage + adjoint gradient; median of 3 runs after one warm-up "
-                      "(%.2f s each, %.1f s of CPU work timed)" % (CPU_SAMPLE_STEPS, t, 4 * t),
+                      "forward with full storage + adjoint gradient; median of 5 runs after one warm-up "
+                      "(%.2f s each, about %.0f s of CPU work timed)" % (CPU_SAMPLE_STEPS, t, 5 * t),
             "small_model": {"value": round(cells_s / ts / 1e9, 4),
                             "sample": "100x200 model + 20-cell pad (248x128 padded, order 8), nt=1000, 1 shot, "
-                                      "forward with full storage + adjoint gradient; median of 3 runs (%.3f s each)"
+                                      "forward with full storage + adjoint gradient; median of 5 runs (%.3f s each)"
                                       % ts}}
 
 
