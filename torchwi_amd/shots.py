@@ -21,16 +21,19 @@ ALIGN = 32  # row stride alignment in floats (128 B)
 GRAPH_STEPS = 100  # time steps per captured hipGraph block (0: direct launches)
 TIME_BLOCK = 1     # time steps per launch of the step kernel (2..4: temporal blocking)
 # Cells per time step (all shots of a plan) from which the temporally blocked
-# kernel with 3 steps per launch is the default: where the three wavefields of
-# all shots (12 B per cell) outgrow MI355X's 256 MB Infinity Cache.  Measured
-# with checkpointed gradients (Gcell-updates/s, one step per launch / blocked):
-# below, the one-step kernel streams from the cache and wins (32 shots of
-# 1032 x 392, 13M cells: 194 vs 145; 48 shots, 19M: 177 vs 154; 3 shots of
-# 3608 x 1208, 13M: 209 vs 159); above, the blocked kernel moves fewer bytes
-# and wins (8 shots of 3008 x 1008, 24M: 168 vs 183; 64 shots of 1032 x 392,
-# 26M: 155 vs 167; 12 shots of 3608 x 1208, 52M: 171 vs 211; 64 shots of
-# 3008 x 1008, 194M: 181 vs 258).
-AUTO_BLOCK_CELLS = 22_000_000
+# kernel with 3 steps per launch is the default.  Below, the three wavefields
+# of all shots (12 B per cell) largely fit MI355X's 256 MB Infinity Cache and
+# the one-step kernel streams from it; above, the blocked kernel, which reads
+# and writes the fields once per 3 steps, wins.  Measured with both kernels'
+# shots in two groups on two streams, nt = 600 (Gcell-updates/s, one step per
+# launch / blocked, checkpointed and full-storage gradients): 32 shots of
+# 1032 x 392, 13M cells: 214 / 194 checkpointed; 3 shots of 3608 x 1208, 13M:
+# 205 / 201 and 285 / 287; 5 shots of 3008 x 1008, 15M: 215 / 233 and
+# 290 / 331; 8 shots of 2408 x 808, 16M: 224 / 231 and 292 / 329; 40 shots of
+# 1032 x 392, 16M: 211 / 209 and 282 / 297; 48 shots, 19M: 185 / 214 and
+# 265 / 305; 8 shots of 3008 x 1008, 24M: 178 / 263; 64 shots of 1032 x 392,
+# 26M: 161 / 224.
+AUTO_BLOCK_CELLS = 15_000_000
 
 
 def auto_time_block(cells_per_step):
