@@ -451,6 +451,35 @@ def test_forward_only_plan_and_autotune():
     assert len(times) == len(CANDIDATES)
 
 
+@pytest.mark.parametrize("order,S,nt,streams", [(8, 5, 40, 0), (4, 3, 31, 1), (2, 4, 29, 2)])
+def test_forward_only_blocked_plan_bitexact(order, S, nt, streams):
+    """A forward-only modelling plan with the blocked kernel (3 steps per
+    launch, no virtual sources, no checkpoints; one or two shot groups) gives
+    the gathers of the one-step forward-only plan bit for bit, and those equal
+    the oracle's."""
+    from torchwi_amd.shots import ShotPlan
+    c = small_case(order, nx=150, ny=60, nt=nt, seed=S * 5 + order)
+    h = c["h"]
+    w = torch.from_numpy(c["w"])
+    vel = torch.from_numpy(c["vel"]).to(DEV)
+    shots = [dict(model=0, sx=(9 + 31 * s + 0.5) * h, sy=(s % 3 + 0.5) * h, ry=(1 + s % 2 + 0.5) * h, rx0=0,
+                  nrec=c["nx"], w=w) for s in range(S)]
+    print(f"case forward-only blocked order={order} S={S} nt={nt} streams={streams}", flush=True)
+    res = []
+    for tb in (3, 1):
+        p = ShotPlan(order, nt, h, c["dt"], [(c["nx"], c["ny"])], shots, 0, DEV, forward_only=True, time_block=tb,
+                     streams=streams)
+        p.models[0].load(vel)
+        p.forward()
+        res.append(p.frd_all().clone())
+        del p
+    assert torch.equal(res[0], res[1]), "blocked vs one-step gathers"
+    vpad = O.pad_vel(c["vel"], order)
+    frd_o, _ = O.forward(vpad, c["w"], order, c["dimx"], c["dimy"], h, c["dt"], shots[1]["sx"], shots[1]["sy"],
+                         shots[1]["ry"], store_virt=False)
+    assert np.array_equal(res[0][1].cpu().numpy(), frd_o)
+
+
 def test_auto_time_block_large_plan_bitexact():
     """A plan above shots.AUTO_BLOCK_CELLS cells per step runs the blocked
     kernel by default (3 steps per launch, many strips and chunks per shot)

@@ -7,6 +7,8 @@ def test_auto_time_block_by_plan_size():
     assert auto_time_block(32 * 1032 * 392) == 1            # 13M cells: one step per launch
     assert auto_time_block(64 * 3008 * 1008) == 3           # 194M cells: blocked kernel
     assert auto_time_block(AUTO_BLOCK_CELLS - 1) == 1 and auto_time_block(AUTO_BLOCK_CELLS) == 3
+    assert auto_time_block(AUTO_BLOCK_CELLS, forward_only=True) == 1             # modelling: one step longer
+    assert auto_time_block(64 * 3008 * 1008, forward_only=True) == 3
 
 
 def test_resolve_time_block_respects_explicit_choices():

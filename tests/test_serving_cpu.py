@@ -57,7 +57,11 @@ def test_chunk_time_block_by_kind_storage_and_size():
     assert chunk_time_block(big, n, 78) == 3 and chunk_time_block(big, n - 1, 78) == 1
     assert chunk_time_block(big, n, 0) == 3            # full storage
     assert chunk_time_block(big, n, 77) == 1           # not a multiple of the block
-    assert chunk_time_block(_req("forward", nx=3600, ny=1200, S=12), 12, 0) == 1
+    from torchwi_amd.shots import AUTO_BLOCK_CELLS_FORWARD
+    fwd = _req("forward", nx=3600, ny=1200, S=12)
+    nf = -(-AUTO_BLOCK_CELLS_FORWARD // cells)
+    assert chunk_time_block(fwd, nf, 0) == 3 and chunk_time_block(fwd, nf - 1, 0) == 1
+    assert nf > n                                       # modelling keeps one step per launch longer
     assert chunk_plan(big, 10 ** 10)[1] % 3 == 0
 
 

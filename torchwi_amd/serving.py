@@ -125,13 +125,15 @@ def chunk_plan(req, cap_bytes):
 
 
 def chunk_time_block(req, nshots, K):
-    """Steps per launch for a plan of nshots shots of req's grid: gradients
-    by plan size (shots.auto_time_block), forward modelling one step per
-    launch (measured faster at every size on MI355X, e.g. 12 shots of
-    3608 x 1208: 355 vs 307 Gcell-updates/s).  Either gives the same bits."""
-    if req.kind != "gradient" or K % 3:
+    """Steps per launch for a plan of nshots shots of req's grid, by plan size
+    (shots.auto_time_block; forward modelling plans with their own
+    threshold).  Either kernel gives the same bits."""
+    cells = nshots * (req.nx + req.order) * (req.ny + req.order)
+    if req.kind != "gradient":
+        return auto_time_block(cells, forward_only=True)
+    if K % 3:
         return 1
-    return auto_time_block(nshots * (req.nx + req.order) * (req.ny + req.order))
+    return auto_time_block(cells)
 
 
 def io_bytes(req):

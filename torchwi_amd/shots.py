@@ -34,11 +34,21 @@ TIME_BLOCK = 1     # time steps per launch of the step kernel (2..4: temporal bl
 # 265 / 305; 8 shots of 3008 x 1008, 24M: 178 / 263; 64 shots of 1032 x 392,
 # 26M: 161 / 224.
 AUTO_BLOCK_CELLS = 15_000_000
+# The same for forward-only modelling plans, which store no virtual sources:
+# the one-step forward moves 16 B per cell-update and keeps the cache longer.
+# Measured (two shot groups, nt = 600, Gcell-updates/s, one step per launch /
+# blocked): 32 shots of 1032 x 392, 13M cells: 491 / 325; 8 shots of
+# 2408 x 808, 16M: 551 / 415; 48 shots of 1032 x 392, 19M: 410 / 367;
+# 8 shots of 3008 x 1008, 24M: 363 / 499; 64 shots of 1032 x 392, 26M:
+# 345 / 398; 12 shots of 3608 x 1208, 52M: 379 / 586; 64 shots of
+# 3008 x 1008, 194M: 351 / 638.
+AUTO_BLOCK_CELLS_FORWARD = 22_000_000
 
 
-def auto_time_block(cells_per_step):
-    """Steps per launch for a plan advancing cells_per_step cells per step."""
-    return 3 if cells_per_step >= AUTO_BLOCK_CELLS else 1
+def auto_time_block(cells_per_step, forward_only=False):
+    """Steps per launch for a plan advancing cells_per_step cells per step
+    (gradient plans, or forward-only modelling plans)."""
+    return 3 if cells_per_step >= (AUTO_BLOCK_CELLS_FORWARD if forward_only else AUTO_BLOCK_CELLS) else 1
 RING = 4           # fields per wavefield ring (3 rotate for one-step launches; 2 pairs for blocks)
 
 
