@@ -149,3 +149,39 @@ def test_serving_grid_classes_alone_equal_mix():
         print(f"case serving grid classes request={i} {spec[i]} alone", flush=True)
         alone = Server(DEV, slots=4, merge=False).run([mk(i)], realtime=False)[0]
         _check_same(alone, mixed[i])
+
+
+def test_cancel_waiting_and_in_flight_requests():
+    """Cancelling a waiting request removes it at once; cancelling one whose
+    chunk is on the device (in a plan it shares with another request) lets
+    that chunk run, launches none of its later chunks and releases its slot
+    and budget; the other requests finish with their results alone, bit for
+    bit."""
+    from torchwi_amd.serving import Server, chunk_plan, make_request
+    from torchwi_amd.shots import default_ckpt_interval, plan_bytes
+    mk = lambda: [make_request("gradient", 40, 96, 150, 4, 12.0, 31, device=DEV, rid=0),  # noqa: E731
+                  make_request("gradient", 40, 96, 150, 4, 12.0, 32, device=DEV, rid=1),
+                  make_request("forward", 40, 96, 150, 3, 12.0, 33, device=DEV, rid=2),
+                  make_request("gradient", 40, 96, 150, 2, 12.0, 34, device=DEV, rid=3)]
+    K = default_ckpt_interval(150, 3)
+    cap = plan_bytes([(96, 40, 8)], [dict(model=0, nrec=96)] * 2, 150, K) + 4096
+    reqs = mk()
+    assert chunk_plan(reqs[1], cap)[0] == 2          # request 1 runs as two chunks of two shots
+    srv = Server(DEV, budget_bytes=BUDGET, slots=2, cap_bytes=cap)
+    res = {r.rid: srv.submit(r) for r in reqs}
+    srv.poll()                                        # admits 0 and 1 (two slots), launches their first chunks
+    assert [w[0].rid for w in srv.waiting] == [2, 3]
+    assert any(m[0].req.rid == 1 for b in srv.batches for m in b.members), "request 1 is not in flight"
+    print("case cancel: waiting request 3, in-flight request 1", flush=True)
+    assert srv.cancel(3) and res[3].cancelled and res[3].error == "cancelled"
+    assert srv.cancel(1) and not srv.cancel(42)
+    while not srv.idle():
+        srv.poll()
+    torch.cuda.synchronize()
+    assert res[1].cancelled and res[1].error == "cancelled" and res[1].chunks == 1
+    assert srv.in_use == 0 and sorted(srv.free_slots) == [0, 1] and not srv.active and not srv.batches
+    for rid in (0, 2):
+        fresh = mk()[rid]
+        alone = Server(DEV, budget_bytes=BUDGET, slots=2, cap_bytes=cap, merge=False).run([fresh],
+                                                                                          realtime=False)[0]
+        _check_same(alone, res[rid])

@@ -26,6 +26,8 @@ request whose chunk fails to build or launch (bad geometry, allocation
 failure, a library error) is ended with ``error`` set: its slot and budget
 are released and the other requests continue; a failed batch of several
 requests is retried request by request to find the one at fault.
+``Server.cancel(rid)`` ends a request the same way: at once while it waits,
+else once its chunk on the device (if any) has run.
 
 Admission is first come first served with bounded skip-ahead: when the
 oldest waiting request does not fit the free budget, a later one that fits
@@ -94,6 +96,7 @@ class Result:
     loss: Optional[float] = None           # gradient
     grad: Optional[torch.Tensor] = None    # gradient: (nx, ny)
     error: Optional[str] = None
+    cancelled: bool = False                # ended by Server.cancel
     chunks: int = 0
     shared_batches: int = 0                # chunks that ran in a batch with other requests
     cell_updates: float = 0.0              # credited work (2 passes for gradients)
@@ -183,7 +186,8 @@ class _Active:
         self.per, self.K, self.cost, self.slot = per, K, cost, slot
         self.next_shot = 0
         self.loss = self.grad = self.frd = self.vel = None
-        self.failed = False
+        self.failed = False      # ended (error or cancel): no further chunk, results dropped
+        self.cancelled = False
 
 
 class _Batch:
@@ -278,16 +282,47 @@ class Server:
         for i in sorted(picked, reverse=True):
             self.waiting.pop(i)
 
+    def cancel(self, rid):
+        """Cancel request rid.  A waiting request leaves the queue at once; an
+        admitted one launches no further chunk and ends, its slot and HBM
+        budget released, as soon as none of its chunks is on the device (a
+        chunk in flight, possibly in a plan shared with other requests, runs
+        to its end; those requests are unaffected).  The result gets
+        cancelled = True and error "cancelled".  Returns False when rid is not
+        waiting or in flight."""
+        for i, w in enumerate(self.waiting):
+            if w[0].rid == rid:
+                self.waiting.pop(i)
+                res = w[1]
+                res.cancelled = True
+                res.error = "cancelled"
+                res.finished = time.perf_counter()
+                self.done.append(res)
+                return True
+        for a in self.active:
+            if a.req.rid == rid and not a.failed:
+                a.cancelled = True
+                if not any(m[0] is a for b in self.batches for m in b.members):
+                    self.due = [x for x in self.due if x is not a]
+                    self._end(a, "cancelled")
+                return True
+        return False
+
     # ---- execution ----------------------------------------------------------
-    def _fail(self, a, err):
+    def _end(self, a, error):
+        """End an admitted request without a result (failure or cancel)."""
         if a.failed:
             return
         a.failed = True
-        a.res.error = "%s: %s" % (type(err).__name__, err)
+        a.res.error = error
+        a.res.cancelled = a.cancelled
         a.res.finished = time.perf_counter()
         if self.verbose:
-            print("serving: request %d failed: %s" % (a.req.rid, a.res.error), file=sys.stderr, flush=True)
+            print("serving: request %d ended: %s" % (a.req.rid, error), file=sys.stderr, flush=True)
         self._release(a)
+
+    def _fail(self, a, err):
+        self._end(a, "%s: %s" % (type(err).__name__, err))
 
     def _release(self, a):
         self.in_use -= a.cost
@@ -370,7 +405,7 @@ class Server:
     def _launch_due(self):
         groups = {}
         for a in self.due:
-            if not a.failed:
+            if not a.failed and not a.cancelled:
                 groups.setdefault(batch_key(a.req, a.K) if self.merge else id(a), []).append(a)
         self.due = []
         for group in groups.values():
@@ -406,7 +441,9 @@ class Server:
             for a, _, _ in b.members:
                 if a.failed:
                     continue
-                if a.next_shot < a.req.nshots:
+                if a.cancelled:
+                    self._end(a, "cancelled")
+                elif a.next_shot < a.req.nshots:
                     self.due.append(a)
                 else:
                     self._finish(a)
