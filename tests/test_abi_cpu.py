@@ -44,6 +44,9 @@ def test_cpu_device_is_refused_not_emulated():
         torchwi.Time2d(20, 10, 10.0, w, 1e-3, 8, "cpu")
     with pytest.raises(NotImplementedError):
         torchwi.Time2dForward(20, 10, 10.0, w, 1e-3, 8, "cpu")
+    for cls in (torchwi.Time2dBatch, torchwi.Time2dForwardBatch):
+        with pytest.raises(NotImplementedError):
+            cls(20, 10, 10.0, w, 1e-3, 8, "cpu")
 
 
 def test_get_operator_surface():

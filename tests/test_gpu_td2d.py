@@ -618,3 +618,29 @@ def test_backward_after_a_later_forward_raises():
         lb1.backward()
     lb2.backward()
     assert np.array_equal(vb.grad.cpu().numpy(), g_o)
+
+
+@pytest.mark.parametrize("order,S,tb", [(8, 4, "auto"), (4, 3, 3), (2, 5, 1)])
+def test_forward_batch_equals_time2dforward_per_shot(order, S, tb):
+    """Time2dForwardBatch (a forward-only plan: one step per launch or the
+    blocked kernel) gives every shot the gather of the drop-in
+    Time2dForward module, bit for bit, and a second call with other
+    positions re-plans."""
+    import torchwi_amd as torchwi
+    c = small_case(order, nx=120, ny=50, nt=90, seed=S * 3 + order)
+    h = c["h"]
+    w = torch.from_numpy(c["w"])
+    vel = torch.from_numpy(c["vel"]).to(DEV)
+    sx = [(5 + 25 * s + 0.5) * h for s in range(S)]
+    sy = [(s % 3 + 0.5) * h for s in range(S)]
+    ry = [(2 + s % 2 + 0.5) * h for s in range(S)]
+    print(f"case forward batch order={order} S={S} time_block={tb}", flush=True)
+    fb = torchwi.Time2dForwardBatch(c["nx"], c["ny"], h, w, c["dt"], order, DEV, time_block=tb)
+    one = torchwi.Time2dForward(c["nx"], c["ny"], h, w, c["dt"], order, DEV)
+    for shift in (0, 1):
+        xs = [x + shift * h for x in sx]
+        frd = fb(vel, xs, sy, ry).clone()
+        assert frd.shape == (S, c["nt"], c["nx"])
+        for s in range(S):
+            g = one(vel, torch.tensor([xs[s]]), torch.tensor([sy[s]]), torch.tensor([ry[s]]))
+            assert torch.equal(frd[s], g), f"shot {s} shift {shift}"

@@ -8,10 +8,12 @@ adjoint gradient, behind TorchWI's operator surface.
     loss.backward()                          # vel.grad (nx, ny)
 
 ``Time2dBatch`` runs many shots in lockstep on one GPU (the replacement for
-the reference's per-process shot loop), with wavefield checkpointing.
+the reference's per-process shot loop), with wavefield checkpointing;
+``Time2dForwardBatch`` models many shots without storing anything for an
+adjoint.
 """
 from torchwi_amd.operator.TimeOperator import Time2d, Time2dForward  # noqa: F401
-from torchwi_amd.batch import Time2dBatch, Time2dWindows  # noqa: F401
+from torchwi_amd.batch import Time2dBatch, Time2dForwardBatch, Time2dWindows  # noqa: F401
 from torchwi_amd.inversion import TimeInversion  # noqa: F401
 
-__all__ = ["Time2d", "Time2dForward", "Time2dBatch", "Time2dWindows", "TimeInversion"]
+__all__ = ["Time2d", "Time2dForward", "Time2dBatch", "Time2dForwardBatch", "Time2dWindows", "TimeInversion"]

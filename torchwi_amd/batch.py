@@ -16,12 +16,15 @@ gradients in shot order.
                        interior column (the reference's acquisition).
 * ``Time2dWindows`` -- S shots, each on its own aperture window cropped
                        from a large model, with its own receiver window.
+* ``Time2dForwardBatch`` -- forward modelling only: S shots on one model in
+                       a plan that stores no virtual sources or checkpoints
+                       (the reference's forward_modeling.py shot loop).
 """
 import torch
 
 from .shots import GRAPH_STEPS, ShotPlan, auto_time_block, default_ckpt_interval, plan_bytes
 
-__all__ = ["Time2dBatch", "Time2dWindows", "default_ckpt_interval"]
+__all__ = ["Time2dBatch", "Time2dWindows", "Time2dForwardBatch", "default_ckpt_interval"]
 
 
 def _as_list(x):
@@ -258,3 +261,63 @@ class Time2dWindows(torch.nn.Module):
             nxw = q["x1"] - q["x0"]
             grad[:, q["x0"]:q["x1"]] += p.grad_of(s)[ne:ne + self.ny, ne:ne + nxw]
         return loss, grad.transpose(0, 1)
+
+
+class Time2dForwardBatch(torch.nn.Module):
+    """Forward modelling of many shots of one velocity model on one GPU.
+
+    The batched replacement of the reference's modelling loop, one
+    ``Time2dForward`` call per shot under Horovod
+    (examples/time/forward_modeling.py:44-52): ``forward(vel, sx, sy, ry)``
+    takes vel (nx, ny) and per-shot positions (length S) and returns the
+    gathers (S, nt, nx), each shot's equal to ``Time2dForward``'s for it bit
+    for bit.  The plan is forward-only (TwPlanSpec.forward_only): no virtual
+    sources, no checkpoints, no adjoint ring, so its memory is the wavefield
+    rings and the gathers (about 5 fields per shot).  There is no gradient
+    (the reference's forward-only operator has none either,
+    TimeFunction.py:24-29).  ``time_block`` "auto" picks the temporally
+    blocked kernel from shots.AUTO_BLOCK_CELLS_FORWARD cells per step.  The
+    returned gathers are the plan's buffer: the next forward overwrites them.
+    """
+
+    def __init__(self, nx, ny, h, w, dt, order, device="cuda", time_block="auto", graph_steps=GRAPH_STEPS,
+                 tile_rows=0, tile_cols=0, streams=0):
+        super().__init__()
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise NotImplementedError("Time2dForwardBatch runs on the GPU only")
+        if order not in (2, 4, 8):
+            raise ValueError("Wrong order: use 2, 4, and 8")
+        self.nx, self.ny, self.order = nx, ny, order
+        self.h, self.dt = float(h), float(dt)
+        self.w = _wavelet(w)
+        self.nt = int(self.w.numel())
+        self.time_block, self.graph_steps = time_block, graph_steps
+        self.tile_rows, self.tile_cols, self.streams = tile_rows, tile_cols, int(streams)
+        self._plan = None
+        self._key = None
+
+    @property
+    def plan(self):
+        return self._plan
+
+    @torch.no_grad()
+    def forward(self, vel, sx, sy, ry):
+        sx, sy, ry = _as_list(sx), _as_list(sy), _as_list(ry)
+        if not (len(sx) == len(sy) == len(ry)) or len(sx) == 0:
+            raise ValueError("sx, sy, ry must have the same positive length")
+        key = (tuple(sx), tuple(sy), tuple(ry))
+        if self._key != key:
+            shots = [dict(model=0, sx=a, sy=b, ry=c, rx0=0, nrec=self.nx, w=self.w) for a, b, c in zip(sx, sy, ry)]
+            cells = len(shots) * (self.nx + self.order) * (self.ny + self.order)
+            tb = (auto_time_block(cells, forward_only=True) if self.time_block == "auto"
+                  else int(self.time_block))
+            self._plan = None
+            torch.cuda.empty_cache()
+            self._plan = ShotPlan(self.order, self.nt, self.h, self.dt, [(self.nx, self.ny)], shots, 0, self.device,
+                                  tile_rows=self.tile_rows, graph_steps=self.graph_steps, time_block=tb,
+                                  forward_only=True, tile_cols=self.tile_cols, streams=self.streams)
+            self._key = key
+        self._plan.models[0].load(vel)
+        self._plan.forward()
+        return self._plan.frd_all()
