@@ -99,7 +99,9 @@ int tw_td2d_exa_backward(float *grad, const float *exa, const short *iexa, float
  * this is the replacement for that loop.  All shots of a plan share order,
  * nt, h and dt; each has its own grid, model, wavelet, source, receiver
  * window and buffers.  Shots march in lockstep: one launch advances every
- * shot by one time step.
+ * shot of the plan (of its shot group, when the plan runs two groups on two
+ * streams, TwPlanSpec.streams) by one time step, or by 2..4 with the
+ * temporally blocked kernel.
  * ---------------------------------------------------------------------- */
 
 typedef struct TwPlanSpec {

@@ -1,8 +1,9 @@
 """Device-resident shot plans: the host side of the batched C-ABI plan.
 
 A ``ShotPlan`` holds every device buffer of a set of shots that march in
-lockstep (one launch per time step for all of them) and the C plan built
-over them.  Shots may live on different grids: each shot names one of the
+lockstep (one launch per time step, or per 3 with the blocked kernel, for
+all of them, or for each of two shot groups on two streams) and the C plan
+built over them.  Shots may live on different grids: each shot names one of the
 plan's models (a padded velocity grid with its derived arrays), so a plan
 can carry many shots of one model (the reference's shot loop,
 examples/time/inv_torch.py:123-143) or one cropped sub-model per shot
