@@ -1249,9 +1249,10 @@ int launch_t_ne(const Plan& p, const TArgs& a, unsigned flags, hipStream_t s, in
 }
 
 // Launch one temporally blocked step of `n` levels starting at time t
-// (advancing by dir), reading field pair in_pair and writing the other.
+// (advancing by dir), reading field pair in_pair (or, with ck_in >= 0, the
+// pair saved for checkpoint segment ck_in) and writing the other pair.
 int launch_block(Plan& p, unsigned flags, int t, int dir, int n, int in_pair, int vslot, int adj,
-                 hipStream_t s, int group = -1)
+                 hipStream_t s, int group = -1, int ck_in = -1)
 {
     TArgs a;
     std::memset(&a, 0, sizeof a);
@@ -1260,6 +1261,7 @@ int launch_block(Plan& p, unsigned flags, int t, int dir, int n, int in_pair, in
     a.nsteps = n;
     a.in_pair = in_pair;
     a.out_pair = in_pair ^ 1;
+    a.ck_in = ck_in;
     a.vslot_first = vslot;
     a.adj = adj;
     a.K = p.K;
@@ -1708,11 +1710,14 @@ int tw_plan_backward(void* plan, int accumulate, void* stream)
             for (int g = 0; g < ng; ++g) {
                 hipStream_t sg = g == 0 ? s : p.s1;
                 const int grp = two ? g : -1;
-                if ((rc = batch_prepare(p, 0, seg, t0, sg, grp))) return rc;
+                // segment 0 starts from zero fields; a later segment's first
+                // launch reads its saved pair in place (no copy into the ring)
+                if (seg == 0 && (rc = batch_prepare(p, 0, 0, 0, sg, grp))) return rc;
                 int b = 0;
                 for (int t = t0; t < t1; t += p.tb, ++b) {
                     const int n = t1 - t < p.tb ? t1 - t : p.tb;
-                    if ((rc = launch_block(p, M_RECOMP, t, +1, n, b & 1, t - t0, 0, sg, grp))) return rc;
+                    const int ck_in = (b == 0 && seg > 0) ? seg : -1;
+                    if ((rc = launch_block(p, M_RECOMP, t, +1, n, b & 1, t - t0, 0, sg, grp, ck_in))) return rc;
                 }
                 int a2 = ab;
                 for (int it = t1 - 1; it >= t0; it -= p.tb, ++a2) {

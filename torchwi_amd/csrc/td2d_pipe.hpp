@@ -41,6 +41,7 @@ struct TArgs {
     int dir;         // +1 forward in time, -1 adjoint
     int nsteps;      // levels to compute (1..T)
     int in_pair, out_pair;  // 0: fields (0,1) of the ring, 1: fields (2,3)
+    int ck_in;       // >= 0: the input pair is checkpoint segment ck_in's saved pair, not ring pair in_pair
     int vslot_first; // virt slot of level 1 (advances with dir)
     int adj;         // 1: adjoint ring
     int K, nseg;     // checkpointing (forward saves)
@@ -141,8 +142,11 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
     const bool lcore = g4 >= c4a && g4 < c4b;
 
     float* const* ring = a.adj ? d.lam : d.u;
-    const float* __restrict__ in_prev = ring[2 * a.in_pair];
-    const float* __restrict__ in_cur = ring[2 * a.in_pair + 1];
+    // the first recomputation launch of a checkpoint segment reads the
+    // segment's saved pair (u[t0-2], u[t0-1]) where it lies
+    const float* __restrict__ in_prev = a.ck_in >= 0 ? d.ckpt + (2LL * a.ck_in) * d.fstride : ring[2 * a.in_pair];
+    const float* __restrict__ in_cur = a.ck_in >= 0 ? d.ckpt + (2LL * a.ck_in + 1) * d.fstride
+                                                    : ring[2 * a.in_pair + 1];
     float* __restrict__ out_prev = ring[2 * a.out_pair];
     float* __restrict__ out_cur = ring[2 * a.out_pair + 1];
 
