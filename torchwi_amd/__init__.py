@@ -7,7 +7,7 @@ adjoint gradient, behind TorchWI's operator surface.
     loss = 0.5 * torch.sum((frd - true) ** 2)
     loss.backward()                          # vel.grad (nx, ny)
 
-``Time2dBatch`` runs many shots in lockstep on one GPU (the replacement for
+``Time2dBatch`` runs many shots on one GPU in one plan (the replacement for
 the reference's per-process shot loop), with wavefield checkpointing;
 ``Time2dForwardBatch`` models many shots without storing anything for an
 adjoint.

@@ -87,7 +87,8 @@ class _BatchOperator(torch.autograd.Function):
 
 
 class Time2dBatch(torch.nn.Module):
-    """Many shots of one velocity model, marched in lockstep on one GPU.
+    """Many shots of one velocity model, marched together on one GPU (in
+    lockstep within each of the plan's shot groups).
 
     ``forward(vel, sx, sy, ry)`` takes vel (nx, ny) and per-shot positions
     (sequences or 1-D tensors of length S) and returns the gathers

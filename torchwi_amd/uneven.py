@@ -3,11 +3,13 @@
     python -m torchwi_amd.uneven [--shots 256] [--nt 6000]
 
 256 shots on a 600 x 4000 model, each on its own aperture window (receiver
-window of 400..2000 cells plus 40 cells each side), all marched in lockstep
-by one plan: one launch per time step for every shot, replayed as hipGraph
-blocks of 100 steps, with the step kernel's tile shape picked by
-torchwi_amd.autotune.  Reports the gradient's throughput and checks two
-shots against the same shots run alone (bitwise).  Prints one JSON line.
+window of 400..2000 cells plus 40 cells each side), all marched by one plan
+(in lockstep within each shot group), with the step kernel picked by
+torchwi_amd.autotune: one step per launch replayed as hipGraph blocks of
+100 steps in one of its tile shapes, or the temporally blocked kernel (3
+steps per direct launch), whichever its probe measures faster.  Reports the
+gradient's throughput and checks two shots against the same shots run alone
+(bitwise).  Prints one JSON line.
 """
 import argparse
 import json
