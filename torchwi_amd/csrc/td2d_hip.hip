@@ -1250,9 +1250,10 @@ int launch_t_ne(const Plan& p, const TArgs& a, unsigned flags, hipStream_t s, in
 
 // Launch one temporally blocked step of `n` levels starting at time t
 // (advancing by dir), reading field pair in_pair (or, with ck_in >= 0, the
-// pair saved for checkpoint segment ck_in) and writing the other pair.
+// pair saved for checkpoint segment ck_in) and writing the other pair (not
+// with no_out: nothing reads it).
 int launch_block(Plan& p, unsigned flags, int t, int dir, int n, int in_pair, int vslot, int adj,
-                 hipStream_t s, int group = -1, int ck_in = -1)
+                 hipStream_t s, int group = -1, int ck_in = -1, int no_out = 0)
 {
     TArgs a;
     std::memset(&a, 0, sizeof a);
@@ -1262,6 +1263,7 @@ int launch_block(Plan& p, unsigned flags, int t, int dir, int n, int in_pair, in
     a.in_pair = in_pair;
     a.out_pair = in_pair ^ 1;
     a.ck_in = ck_in;
+    a.no_out = no_out;
     a.vslot_first = vslot;
     a.adj = adj;
     a.K = p.K;
@@ -1717,7 +1719,11 @@ int tw_plan_backward(void* plan, int accumulate, void* stream)
                 for (int t = t0; t < t1; t += p.tb, ++b) {
                     const int n = t1 - t < p.tb ? t1 - t : p.tb;
                     const int ck_in = (b == 0 && seg > 0) ? seg : -1;
-                    if ((rc = launch_block(p, M_RECOMP, t, +1, n, b & 1, t - t0, 0, sg, grp, ck_in))) return rc;
+                    // the segment's last launch leaves no fields: the adjoint
+                    // needs only its virtual sources
+                    const int no_out = t + p.tb >= t1;
+                    if ((rc = launch_block(p, M_RECOMP, t, +1, n, b & 1, t - t0, 0, sg, grp, ck_in, no_out)))
+                        return rc;
                 }
                 int a2 = ab;
                 for (int it = t1 - 1; it >= t0; it -= p.tb, ++a2) {

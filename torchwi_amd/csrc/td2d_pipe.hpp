@@ -42,6 +42,7 @@ struct TArgs {
     int nsteps;      // levels to compute (1..T)
     int in_pair, out_pair;  // 0: fields (0,1) of the ring, 1: fields (2,3)
     int ck_in;       // >= 0: the input pair is checkpoint segment ck_in's saved pair, not ring pair in_pair
+    int no_out;      // 1: no launch follows that reads the outgoing pair (a segment's last recomputation)
     int vslot_first; // virt slot of level 1 (advances with dir)
     int adj;         // 1: adjoint ring
     int K, nseg;     // checkpointing (forward saves)
@@ -447,7 +448,7 @@ __global__ __launch_bounds__(64 * (T + 1)) void k_pipe(const ShotDesc* __restric
                     // ones, 64 shots of 3008 x 1008, us per launch: forward
                     // 995 vs 1021, recomputation 1531 vs 1580, adjoint 1663
                     // vs 1761).
-                    if (k == a.nsteps) {
+                    if (k == a.nsteps && !a.no_out) {
                         gst4_nt(out_prev + i0, C);
                         gst4_nt(out_cur + i0, un);
                     }
