@@ -57,8 +57,10 @@ constexpr int PD = 2;      // prefetch distance of the adjoint in iterations (< 
 // k-1 by LAG = 4 + BI rows, so that each row a level reads was written at
 // least one barrier earlier, and the LDS rings are deep enough for the waves
 // of a workgroup to be up to BI - 1 iterations apart between barriers.
-// BI = 2 (a barrier every two rows) measured 2% faster than BI = 1 on 64
-// shots of 3008 x 1008 and 22% slower on 192 shots of 1032 x 392.
+// BI = 2 (a barrier every two rows, deeper LDS rings and fewer resident
+// workgroups) measured 13-16% slower than BI = 1 for the checkpointed gradient
+// of 64 shots of 3008 x 1008 (10.9-11.2 vs 9.6 s), and 22% slower on 192
+// shots of 1032 x 392.
 constexpr int BI = 1;           // iterations per workgroup barrier
 constexpr int LAG = 4 + BI;     // rows between consecutive levels: stencil radius + BI
 constexpr int NCUR = LAG - 4 + BI;  // LDS slots of the newest rows of a level
