@@ -367,6 +367,37 @@ def test_temporal_blocking_bitexact(order, T, nx, ny, nt, K, rows):
     assert np.array_equal(res[1][0][1].cpu().numpy(), frd_o)
 
 
+@pytest.mark.parametrize("streams,nt,K", [(1, 50, 6), (2, 50, 6), (2, 37, 9), (1, 20, 30)])
+def test_blocked_backward_twice_after_one_forward(streams, nt, K):
+    """A checkpointed blocked plan's forward pass stores the last segment's
+    virtual sources in the segment buffer, and its backward adjoints that
+    segment without recomputing it.  A second backward after the same
+    forward finds the buffer holding another segment's and recomputes every
+    segment: both give the per-shot gradients of one step per launch, bit for
+    bit (K = 30 > nt: one segment, stored whole by the forward)."""
+    from torchwi_amd import Time2dBatch
+    order, nx, ny, S = 8, 120, 60, 3
+    c = small_case(order, nx=nx, ny=ny, nt=nt, seed=nt + K)
+    sx = [(i + 0.5) * c["h"] for i in (2, nx // 2, nx - 3)]
+    sy = [(i + 0.5) * c["h"] for i in (0, ny // 3, ny - 2)]
+    ry = [(i + 0.5) * c["h"] for i in (1, 0, ny // 2)]
+    print(f"case streams={streams} nt={nt} K={K}", flush=True)
+    vel = torch.from_numpy(c["vel"]).to(DEV)
+    w = torch.from_numpy(c["w"])
+    ref = Time2dBatch(nx, ny, c["h"], w, c["dt"], order, DEV, ckpt_every=0, time_block=1)
+    d_obs = 0.6 * ref.modeling(vel, sx, sy, ry).clone()
+    ref.misfit_gradient(vel, sx, sy, ry, d_obs)
+    want = [ref.plan.grad_of(s).clone() for s in range(S)]
+    m = Time2dBatch(nx, ny, c["h"], w, c["dt"], order, DEV, ckpt_every=K, time_block=3, streams=streams)
+    m.misfit_gradient(vel, sx, sy, ry, d_obs)
+    first = [m.plan.grad_of(s).clone() for s in range(S)]
+    m.plan.backward()
+    second = [m.plan.grad_of(s).clone() for s in range(S)]
+    for s in range(S):
+        assert torch.equal(first[s], want[s]), f"shot {s}: first backward"
+        assert torch.equal(second[s], want[s]), f"shot {s}: second backward"
+
+
 @pytest.mark.skip(reason="grad_group > 1 is switched off: its first MI355X run (case 8-5-2-0-100) ended in a "
                          "GPU memory fault whose cause has not been found; the plan refuses grad_group > 1")
 @pytest.mark.parametrize("order,S,G,K,graph", [

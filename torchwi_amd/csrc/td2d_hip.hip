@@ -653,6 +653,9 @@ struct Plan {
     int static_dev = -1;              // >= 0: the descriptors live in that device's static workspace
     hipStream_t last_stream = nullptr;  // stream of the latest forward / backward pass
     bool used = false;
+    // blocked checkpointed plans: the segment whose virtual sources the latest
+    // forward pass left in the segment buffer (-1: none valid)
+    int virt_seg = -1;
     ~Plan()
     {
         for (hipEvent_t e : ev)
@@ -1607,29 +1610,33 @@ int tw_plan_forward(void* plan, void* stream)
     if ((rc = prep_models(p, s))) return rc;
     // zero the forward ring: prepare(seg 0) zeroes two fields, the third is
     // written completely by the first step.
+    p.virt_seg = -1;
     if ((rc = batch_prepare(p, 0, 0, 0, s))) return rc;
     const bool store = p.K == 0 && !p.fwd_only;
     const int kind = store ? TW_KIND_FWD_VIRT : TW_KIND_FWD;
     if (p.tb > 1) {
         const unsigned flags = store ? M_FWD_VIRT : M_FWD;
-        if (p.ngroups == 2) {
-            // the two groups march independently, one per stream
-            if ((rc = group_streams(p)) || (rc = join(p, 0, s, p.s1))) return rc;
-            for (int g = 0; g < 2; ++g) {
-                hipStream_t sg = g == 0 ? s : p.s1;
-                int b = 0;
-                for (int t0 = 0; t0 < p.nt; t0 += p.tb, ++b) {
-                    const int n = p.nt - t0 < p.tb ? p.nt - t0 : p.tb;
-                    if ((rc = launch_block(p, flags, t0, +1, n, b & 1, t0, 0, sg, g))) return rc;
-                }
+        // A checkpointed gradient plan's last segment also stores its virtual
+        // sources into the segment buffer (slot t - tlast), so that the
+        // backward starts with that segment's adjoint and recomputes nothing
+        // for it (tlast is a multiple of the time block, K being one).
+        const int tlast = (!store && !p.fwd_only && p.K > 0) ? (p.nseg - 1) * p.K : -1;
+        const int ng = p.ngroups == 2 ? 2 : 1;
+        // with two groups, the two march independently, one per stream
+        if (ng == 2 && ((rc = group_streams(p)) || (rc = join(p, 0, s, p.s1)))) return rc;
+        for (int g = 0; g < ng; ++g) {
+            hipStream_t sg = g == 0 ? s : p.s1;
+            int b = 0;
+            for (int t0 = 0; t0 < p.nt; t0 += p.tb, ++b) {
+                const int n = p.nt - t0 < p.tb ? p.nt - t0 : p.tb;
+                const bool v = tlast >= 0 && t0 >= tlast;
+                if ((rc = launch_block(p, v ? M_FWD_VIRT : flags, t0, +1, n, b & 1, v ? t0 - tlast : t0, 0, sg,
+                                       ng == 2 ? g : -1)))
+                    return rc;
             }
-            return join(p, 1, p.s1, s);
         }
-        int b = 0;
-        for (int t0 = 0; t0 < p.nt; t0 += p.tb, ++b) {
-            const int n = p.nt - t0 < p.tb ? p.nt - t0 : p.tb;
-            if ((rc = launch_block(p, flags, t0, +1, n, b & 1, t0, 0, s))) return rc;
-        }
+        if (ng == 2 && (rc = join(p, 1, p.s1, s))) return rc;
+        if (tlast >= 0) p.virt_seg = p.nseg - 1;
         return TW_OK;
     }
     // one step per launch; with two shot groups, group g on stream sg[g]
@@ -1704,7 +1711,11 @@ int tw_plan_backward(void* plan, int accumulate, void* stream)
         // independently, so one group's memory-bound adjoint overlaps the
         // other's recomputation wherever the two drift apart.  Measured on
         // MI355X (64 shots of 3008 x 1008, nt = 1800): 2.284 s, against
-        // 2.318 s with the groups held in lockstep half a segment apart.
+        // 2.318 s with the groups held in lockstep half a segment apart.  The
+        // segment whose virtual sources the forward pass stored (the last)
+        // needs no recomputation; the next recomputation overwrites them.
+        const int have = p.virt_seg;
+        p.virt_seg = -1;
         int ab = 0;
         for (int seg = p.nseg - 1; seg >= 0; --seg) {
             const int t0 = seg * p.K;
@@ -1714,9 +1725,9 @@ int tw_plan_backward(void* plan, int accumulate, void* stream)
                 const int grp = two ? g : -1;
                 // segment 0 starts from zero fields; a later segment's first
                 // launch reads its saved pair in place (no copy into the ring)
-                if (seg == 0 && (rc = batch_prepare(p, 0, 0, 0, sg, grp))) return rc;
+                if (seg != have && seg == 0 && (rc = batch_prepare(p, 0, 0, 0, sg, grp))) return rc;
                 int b = 0;
-                for (int t = t0; t < t1; t += p.tb, ++b) {
+                for (int t = seg == have ? t1 : t0; t < t1; t += p.tb, ++b) {
                     const int n = t1 - t < p.tb ? t1 - t : p.tb;
                     const int ck_in = (b == 0 && seg > 0) ? seg : -1;
                     // the segment's last launch leaves no fields: the adjoint
