@@ -171,12 +171,14 @@ int tw_plan_create(const TwPlanSpec *spec, const TwShotSpec *shots, int nshots,
                    void *workspace, size_t workspace_bytes, void *stream, void **plan);
 
 /* Forward pass of every shot: fills frd, and virt (full storage) or the
- * checkpoints. */
+ * checkpoints (with time_block > 1 also the last segment's virtual sources,
+ * which the next tw_plan_backward uses instead of recomputing them). */
 int tw_plan_forward(void *plan, void *stream);
 
 /* Adjoint pass of every shot (recomputing each checkpoint segment first when
  * ckpt_every > 0): grad = (accumulate ? grad : 0) + sum_t -virt[t]*lambda[t]
- * per shot, in its own grad buffer. */
+ * per shot, in its own grad buffer.  May be repeated after one forward pass
+ * (every segment is then recomputed). */
 int tw_plan_backward(void *plan, int accumulate, void *stream);
 
 void tw_plan_destroy(void *plan);
