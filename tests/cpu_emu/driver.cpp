@@ -15,6 +15,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 thread_local uint3_ threadIdx, blockIdx, blockDim, gridDim;
@@ -111,6 +112,20 @@ int main(int argc, char** argv)
     if (rc0 || tw_plan_forward(plan, nullptr) || tw_plan_backward(plan, 0, nullptr)) {
         fprintf(stderr, "plan: %s\n", tw_last_error());
         return 1;
+    }
+    // TW_EMU_BACKWARD_TWICE=1: a second backward after the same forward must
+    // give the same gradients bit for bit (it recomputes every segment)
+    const char* bt = getenv("TW_EMU_BACKWARD_TWICE");
+    if (bt && bt[0] == '1') {
+        const std::vector<float> first = grad;
+        if (tw_plan_backward(plan, 0, nullptr)) {
+            fprintf(stderr, "second backward: %s\n", tw_last_error());
+            return 1;
+        }
+        if (std::memcmp(first.data(), grad.data(), sizeof(float) * grad.size()) != 0) {
+            fprintf(stderr, "second backward differs from the first\n");
+            return 3;
+        }
     }
     tw_plan_destroy(plan);
     FILE* g = fopen(argv[2], "wb");

@@ -54,6 +54,21 @@ def driver(tmp_path_factory):
     (8, 20, 12, 10, 5, 0, 4, 1, 0, 2), (4, 37, 15, 9, 3, 4, 3, 1, 0, 2), (2, 30, 14, 11, 4, 3, 0, 1, 0, 2)])
 def test_kernel_source_on_host_threads_matches_oracle(driver, tmp_path, order, nx, ny, nt, S, K, graph, tb, rows,
                                                       static):
+    run_case(driver, tmp_path, order, nx, ny, nt, S, K, graph, tb, rows, static)
+
+
+@pytest.mark.parametrize("order,nx,ny,nt,S,K,graph,tb,rows,static", [
+    (8, 24, 14, 10, 3, 3, 0, 3, 0, 2), (4, 40, 22, 14, 2, 4, 0, 2, 0, 0), (8, 20, 12, 9, 2, 30, 0, 3, 0, 0),
+    (4, 37, 15, 9, 3, 4, 3, 1, 0, 2)])
+def test_second_backward_on_host_threads(driver, tmp_path, order, nx, ny, nt, S, K, graph, tb, rows, static):
+    """A second backward after one forward (the blocked forward pass left the
+    last segment's virtual sources, which the first backward used and
+    overwrote) gives the first backward's gradients bit for bit, and both
+    the oracle's; K = 30 > nt: one segment, stored whole by the forward."""
+    run_case(driver, tmp_path, order, nx, ny, nt, S, K, graph, tb, rows, static, twice=True)
+
+
+def run_case(driver, tmp_path, order, nx, ny, nt, S, K, graph, tb, rows, static, twice=False):
     rng = np.random.default_rng(order * 7 + nx)
     h, dt = 10.0, 1e-3
     vel = (1500.0 + 2500.0 * rng.random((nx, ny))).astype(np.float32)
@@ -73,7 +88,7 @@ def test_kernel_source_on_host_threads_matches_oracle(driver, tmp_path, order, n
         for a in (vld, w, pos, resid):
             a.tofile(f)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", TW_EMU_STATIC=str(int(static == 1)),
-               TW_EMU_STREAMS=str(2 if static == 2 else 0))
+               TW_EMU_STREAMS=str(2 if static == 2 else 0), TW_EMU_BACKWARD_TWICE=str(int(twice)))
     r = subprocess.run([driver, fin, fout], capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     out = np.fromfile(fout, np.float32)
